@@ -1,0 +1,273 @@
+"""Benchmark: differentiable MLS-MPM substep (forward + adjoint) on one MI355X.
+
+Metric: particle-substeps/s, forward+backward (one particle advanced one substep and
+its adjoint substep), batched uneven scenes vs the same scenes run one by one through
+the single-scene path.
+
+Workload (default, "batched"): 64 scenes, particle counts U[5k, 60k], 64^3 grid each,
+yield stress U[20, 100], 2-22 capsules per scene with constant random velocities,
+100 substeps forward then 100 adjoint substeps, all scenes in one launch per stage.
+A step = one such rollout (forward + backward) of all 64 scenes.
+
+Prints one JSON line (rank 0). --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per particle per kernel launch (state, material and temporaries the
+# kernel must read or write; grid traffic excluded). See the design notes.
+KERNEL_BYTES = {
+    "p2g_fwd": 152,      # read x,v,F,C + mass,vol,mu/lam/yield (116) ; write F' (36)
+    "grid_fwd": 0,
+    "g2p_fwd": 72,       # read x (12) ; write x',v',C' (60)
+    "p2g_recompute": 236,  # read x,v,F,C + material (116) ; write F_tmp,U,V,sigma (120)
+    "grid_recompute": 0,
+    "g2p_adj": 108,      # read x,v',dx',dv',dC' (84) ; read+write dx (24)
+    "grid_adj": 0,
+    "p2g_adj": 464,      # read x,v,F,C,material,temps,dF' (272) ; read+write dx,dv,dF,dC (192)
+}
+FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
+BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_batched(n_scenes, seed, T):
+    from dexdeform_amd.mpm import workloads as W
+    scenes = W.batched_scenes(n_scenes=n_scenes, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    poses = []
+    for sc in scenes:
+        vel = rng.uniform(-0.5, 0.5, size=(sc.nb, 3)).astype(np.float32)
+        t = np.arange(T + 1, dtype=np.float32)[:, None, None] * sc.spec.dt
+        pos = sc.body_pos[None] + t * vel[None]
+        rot = np.broadcast_to(sc.body_rot[None], (T + 1, sc.nb, 4))
+        poses.append((pos.astype(np.float32), np.ascontiguousarray(rot, np.float32)))
+    return scenes, poses
+
+
+def build_rollout(scenes, poses, T, device, checkpoint_every=None):
+    import torch
+    from dexdeform_amd.mpm.engine import Engine
+    from dexdeform_amd.mpm.rollout import Rollout
+    eng = Engine([s.spec for s in scenes], device=device, with_grad=True)
+    ro = Rollout(eng, T, checkpoint_every=checkpoint_every)
+    x0 = torch.tensor(np.concatenate([s.x0 for s in scenes]), device=device)
+    ro.set_initial(x0)
+    if eng.n_bodies_total:
+        ro.set_poses(torch.tensor(np.concatenate([p[0] for p in poses], 1), device=device),
+                     torch.tensor(np.concatenate([p[1] for p in poses], 1), device=device))
+    rng = np.random.Generator(np.random.PCG64(11))
+    gx = torch.tensor(rng.normal(size=(eng.n_particles_total, 3)).astype(np.float32) / eng.n_particles_total,
+                      device=device)
+    return eng, ro, gx
+
+
+def run_step(ro, gx):
+    ro.forward()
+    ro.backward(x_grad=gx)
+
+
+class KernelTimer:
+    """Per-kernel durations from hipEvents recorded between the stage kernels of one
+    rollout, on the stream the kernels run on."""
+
+    def __init__(self, ro):
+        import torch
+        self.ro = ro
+        self.torch = torch
+
+    def timed_step(self, gx):
+        import ctypes
+        from dexdeform_amd import _lib
+        torch = self.torch
+        ro = self.ro
+        eng = ro.engine
+        lib = eng.lib
+        st = _lib.stream_ptr()
+        fwd_ev, bwd_ev = [], []
+
+        def events(n):
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+            for e in evs:
+                e.record()  # creates the underlying hipEvent
+            arr = (ctypes.c_void_p * n)(*[e.cuda_event for e in evs])
+            return evs, arr
+
+        assert ro.full, "kernel timing uses the non-checkpointed rollout"
+        for t in range(ro.T):
+            evs, arr = events(4)
+            _lib.check(lib.ddm_substep_forward_timed(ctypes.byref(eng._batch),
+                                                     ctypes.byref(ro._sview(t, t).ctype()),
+                                                     ctypes.byref(ro._sview(t + 1, t + 1).ctype()), st, arr))
+            fwd_ev.append((evs, arr))
+        gn = ro.g[0]
+        gn.x.copy_(gx)
+        gn.v.zero_()
+        gn.F.zero_()
+        gn.C.zero_()
+        ro.pos_grad.zero_()
+        ro.rot_grad.zero_()
+        cur = 0
+        for t in range(ro.T - 1, -1, -1):
+            nxt, cur = cur, 1 - cur
+            ro.g[cur].zero_()
+            evs, arr = events(6)
+            _lib.check(lib.ddm_substep_backward_timed(
+                ctypes.byref(eng._batch), ctypes.byref(ro._sview(t, t).ctype()),
+                ctypes.byref(ro._sview(t + 1, t + 1).ctype()), ctypes.byref(ro._gview(cur, t).ctype()),
+                ctypes.byref(ro._gview(nxt, t + 1).ctype()), st, arr))
+            bwd_ev.append((evs, arr))
+        torch.cuda.synchronize()
+        tot = {k: 0.0 for k in FWD_NAMES + BWD_NAMES}
+        for evs, _ in fwd_ev:
+            for i, k in enumerate(FWD_NAMES):
+                tot[k] += evs[i].elapsed_time(evs[i + 1])
+        for evs, _ in bwd_ev:
+            for i, k in enumerate(BWD_NAMES):
+                tot[k] += evs[i].elapsed_time(evs[i + 1])
+        return {k: v / ro.T for k, v in tot.items()}  # ms per launch
+
+
+def cpu_baseline(budget_s=20.0):
+    """The oracle (OpenMP build) on the plain 10k-particle scene: forward + adjoint
+    substeps, timed on the host cores."""
+    from oracle import oracle as O
+    from dexdeform_amd.mpm import workloads as W
+    sc = W.plain_scene(n_particles=10000)
+    s = sc.spec
+    osc = O.Scene(grid_dim=tuple(s.grid_dim), dx=s.dx, dt=s.dt, gravity=tuple(s.gravity), mass=s.mass, vol=s.vol,
+                  mly=s.mu_lam_yield, tfsr=s.body_prim, args=s.body_args)
+    osim = O.OracleSim(osc, np.float32, omp=True)
+    n = sc.n
+    st = O.State(sc.x0, np.zeros((n, 3), np.float32), np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)),
+                 np.zeros((n, 3, 3), np.float32), sc.body_pos, sc.body_rot)
+    steps = 0
+    t0 = time.perf_counter()
+    states = [st]
+    while time.perf_counter() - t0 < budget_s / 2 and steps < 200:
+        states.append(osim.substep(states[-1], sc.body_pos, sc.body_rot))
+        steps += 1
+    g = O.Grads(np.ones((n, 3), np.float32) / n, np.zeros((n, 3), np.float32), np.zeros((n, 3, 3), np.float32),
+                np.zeros((n, 3, 3), np.float32), np.zeros((2, 3), np.float32), np.zeros((2, 4), np.float32))
+    gs = [O.Grads.zeros(n, 2, np.float32) for _ in range(steps + 1)]
+    gs[-1] = g
+    for t in range(steps - 1, -1, -1):
+        osim.substep_grad(states[t], states[t + 1], gs[t + 1], gs[t])
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-substeps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"oracle OpenMP fp32, plain scene 10k particles 64^3 2 capsules, {steps} substeps forward "
+                      f"+ {steps} adjoint substeps ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--substeps", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    dev = torch.device("cuda:0")
+    T = args.substeps
+    scenes, poses = make_batched(args.scenes, args.seed, T)
+    n_total = sum(s.n for s in scenes)
+    log(f"batched: {len(scenes)} scenes, {n_total} particles, {sum(s.nb for s in scenes)} bodies, T={T}")
+
+    # ---------------- batched
+    eng, ro, gx = build_rollout(scenes, poses, T, dev)
+    for _ in range(args.warmup):
+        run_step(ro, gx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        run_step(ro, gx)
+        torch.cuda.synchronize()
+        log(f"batched step {i + 1}/{args.steps} done")
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n_total * T / ((t1 - t0) / args.steps)
+    kt = KernelTimer(ro).timed_step(gx)
+    log("per-kernel ms/launch: " + ", ".join(f"{k}={v:.4f}" for k, v in kt.items()))
+    bytes_per = {k: KERNEL_BYTES[k] * n_total for k in kt}
+    dom = max(kt, key=kt.get)
+    achieved = bytes_per[dom] / (kt[dom] * 1e-3) / 1e9 if kt[dom] > 0 else 0.0
+    sum_ms = sum(kt.values())
+    alg = 480.0 * n_total / (sum_ms * 1e-3) / 1e9
+
+    # ---------------- plain: the same scenes one at a time through the single-scene path
+    plain_value = None
+    if not args.no_plain:
+        tot_t, tot_ps = 0.0, 0
+        for i, (sc, po) in enumerate(zip(scenes, poses)):
+            e1, r1, g1 = build_rollout([sc], [po], T, dev)
+            run_step(r1, g1)
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            run_step(r1, g1)
+            torch.cuda.synchronize()
+            tot_t += time.perf_counter() - a
+            tot_ps += sc.n * T
+            del e1, r1, g1
+            if (i + 1) % 16 == 0:
+                log(f"plain {i + 1}/{len(scenes)} scenes done")
+        plain_value = tot_ps / tot_t
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(args.cpu_budget)
+
+    out = {
+        "metric": "particle-substeps/s fwd+bwd, batched vs plain at <=1e-4 err; p50/p99 rollout ms",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched uneven scenes: {len(scenes)} scenes, particles U[5k,60k] "
+                               f"(total {n_total}), 64^3 grid each, 2-22 capsules, {T} substeps fwd+bwd",
+                   "global_batch": len(scenes), "substeps": T, "particles": n_total, "parallelism": "none"},
+        "plain_value": plain_value,
+        "batched_over_plain": (value / plain_value) if plain_value else None,
+        "kernel_ms_per_launch": kt,
+        "algorithmic_fwd_bwd_GBps": alg,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

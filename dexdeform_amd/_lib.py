@@ -1,0 +1,135 @@
+"""ctypes binding of libdexdeform_mpm.so (the C ABI declared in include/dexdeform_mpm.h).
+
+The library is loaded after ``import torch`` so that its NEEDED ``libamdhip64.so.7``
+resolves to the HIP runtime PyTorch-ROCm already loaded; streams and device pointers
+from torch are therefore valid in the library. There is no fallback: if the library is
+missing or fails to load, every simulator entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch  # noqa: F401  (must precede loading the HIP library)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libdexdeform_mpm.so")
+
+c_int32 = ctypes.c_int32
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+
+DDM_EINVAL = 10001
+DDM_ETOOMANYBODIES = 10002
+DDM_MAX_BODIES = 64
+
+ERROR_NAMES = {DDM_EINVAL: "DDM_EINVAL (invalid argument)", DDM_ETOOMANYBODIES: "DDM_ETOOMANYBODIES"}
+
+
+class DdmScene(ctypes.Structure):
+    _fields_ = [
+        ("p_begin", c_int32), ("p_count", c_int32), ("blk_begin", c_int32), ("n_blocks", c_int32),
+        ("dim", c_int32 * 3), ("n_bodies", c_int32), ("b_begin", c_int32), ("adjoint_mode", c_int32),
+        ("pad0", c_int32), ("pad1", c_int32),
+        ("dx", c_float), ("inv_dx", c_float), ("dt", c_float), ("ground_friction", c_float),
+        ("ground_height", c_float), ("gravity", c_float * 3),
+    ]
+
+
+SCENE_DTYPE = np.dtype([
+    ("p_begin", "<i4"), ("p_count", "<i4"), ("blk_begin", "<i4"), ("n_blocks", "<i4"),
+    ("dim", "<i4", (3,)), ("n_bodies", "<i4"), ("b_begin", "<i4"), ("adjoint_mode", "<i4"),
+    ("pad0", "<i4"), ("pad1", "<i4"),
+    ("dx", "<f4"), ("inv_dx", "<f4"), ("dt", "<f4"), ("ground_friction", "<f4"),
+    ("ground_height", "<f4"), ("gravity", "<f4", (3,)),
+])
+assert SCENE_DTYPE.itemsize == ctypes.sizeof(DdmScene) == 80
+
+
+class DdmBatch(ctypes.Structure):
+    _fields_ = [
+        ("n_scenes", c_int32), ("n_chunks", c_int32), ("n_blocks", c_int32), ("max_bodies", c_int32),
+        ("scenes", c_void_p), ("chunks", c_void_p), ("mass", c_void_p), ("vol", c_void_p),
+        ("mu_lam_yield", c_void_p), ("body_prim", c_void_p), ("body_args", c_void_p),
+        ("grid_mmv", c_void_p), ("grid_v", c_void_p), ("grid_dv", c_void_p), ("grid_dmmv", c_void_p),
+        ("block_flags", c_void_p), ("tmp_Ft", c_void_p), ("tmp_U", c_void_p), ("tmp_V", c_void_p),
+        ("tmp_sig", c_void_p),
+    ]
+
+
+class DdmState(ctypes.Structure):
+    _fields_ = [("x", c_void_p), ("v", c_void_p), ("F", c_void_p), ("C", c_void_p),
+                ("body_pos", c_void_p), ("body_rot", c_void_p)]
+
+
+# Symbols declared in include/dexdeform_mpm.h
+REFERENCE_SYMBOLS = ["compute_svd", "compute_svd_grad", "p2g", "p2g_grad", "grid_op_v2", "grid_op_v2_grad", "g2p",
+                     "g2p_grad", "compute_dist", "particle2mass"]
+FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_forward_timed",
+                 "ddm_substep_backward_timed", "ddm_batch_reset", "ddm_version"]
+ALL_SYMBOLS = REFERENCE_SYMBOLS + FUSED_SYMBOLS
+
+_P = c_void_p
+_I = ctypes.c_int
+_F = c_float
+_ARGTYPES = {
+    "compute_svd": [_P, _P, _P, _P, _P, _P, _F, _I, _P],
+    "compute_svd_grad": [_P] * 11 + [_F, _I, _P],
+    "p2g": [_P] * 11 + [_P, _F, _F, _F, _P, _P, _P, _I, _P],
+    "p2g_grad": [_P] * 11 + [_P, _F, _F, _F] + [_P] * 13 + [_I, _P],
+    "grid_op_v2": [_P] * 11 + [_F, _F, _F, _F, _P, _P, _I, _P],
+    "grid_op_v2_grad": [_P] * 17 + [_F, _F, _F, _F, _P, _P, _P, _I, _P],
+    "g2p": [_P, _P, _P, _F, _F, _F, _P, _P, _F, _P, _P, _I, _P],
+    "g2p_grad": [_P, _P, _P, _F, _F, _F, _P, _P, _F, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+    "compute_dist": [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _I, _P],
+    "particle2mass": [_P, _P, _P, _P, _F, _F, _P, _P, _P, _P, _I, _I, _I, _P],
+    "ddm_substep_forward": [ctypes.POINTER(DdmBatch), ctypes.POINTER(DdmState), ctypes.POINTER(DdmState), _P],
+    "ddm_substep_backward": [ctypes.POINTER(DdmBatch)] + [ctypes.POINTER(DdmState)] * 4 + [_P],
+    "ddm_substep_forward_timed": [ctypes.POINTER(DdmBatch), ctypes.POINTER(DdmState), ctypes.POINTER(DdmState),
+                                  _P, _P],
+    "ddm_substep_backward_timed": [ctypes.POINTER(DdmBatch)] + [ctypes.POINTER(DdmState)] * 4 + [_P, _P],
+    "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],
+    "ddm_version": [],
+}
+
+_lib = None
+
+
+class ExtensionMissing(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (cached). Raises ExtensionMissing if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ExtensionMissing(
+            f"{path} is not built; run `python -m dexdeform_amd.build` (the MI355X kernels have no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, args in _ARGTYPES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str = "kernel") -> None:
+    if status != 0:
+        name = ERROR_NAMES.get(status, f"hipError {status}")
+        raise RuntimeError(f"dexdeform_mpm {what} failed: {name}")
+
+
+def stream_ptr(stream=None) -> int:
+    """Raw hipStream_t of a torch stream (default: the current stream)."""
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def ptr(t) -> int:
+    """Device address of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return int(t.data_ptr())

@@ -1,0 +1,72 @@
+"""Builds libdexdeform_mpm.so (HIP kernels + C ABI) for gfx950 with hipcc.
+
+Usage: ``python -m dexdeform_amd.build`` (or ``dexdeform_amd.build.build()``).
+The library is written in-tree to ``dexdeform_amd/lib/`` so that it travels with the
+source tree; it is loaded after ``import torch`` so that it binds to the same HIP
+runtime as PyTorch-ROCm.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIBNAME = "libdexdeform_mpm.so"
+SOURCES = ["stage_kernels.hip", "fused_kernels.hip"]
+ARCH = os.environ.get("DDM_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # hardware FP32 atomic add on global memory (no CAS loop)
+    "-munsafe-fp-atomics",
+    # keep the compiler from forming packed FP32 (v_pk_*_f32) instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(p):
+        raise RuntimeError("hipcc not found; the MI355X extension cannot be built")
+    return p
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source and link the shared library. Always recompiles."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        _run(cmd, verbose)
+        objs.append(obj)
+    out = os.path.join(LIBDIR, LIBNAME)
+    tmp = out + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{out}")
+    if verbose and out:
+        print(out)
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

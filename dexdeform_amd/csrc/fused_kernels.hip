@@ -1,0 +1,576 @@
+// Fused multi-scene substep for gfx950.
+//
+// Layout and invariants
+//  * Particles of all scenes are concatenated; a chunk table maps each workgroup to
+//    <= 256 particles of one scene.
+//  * Each scene's grid is stored in 4x4x4 node blocks (64 nodes = one wavefront),
+//    blocks of all scenes concatenated; every grid array is float4 per node.
+//  * Block-sparse: P2G marks the blocks it touches in block_flags; the grid kernels
+//    visit only flagged blocks. Between substeps grid_mmv, grid_dv and block_flags are
+//    all zero: the last grid kernel of a substep clears what it consumed, so no dense
+//    memset is ever needed.
+//  * P2G and the G2P adjoint scatter into an LDS tile covering the bounding box of the
+//    chunk's stencils and flush only the non-zero tile nodes with global atomics; when
+//    the box exceeds the tile budget the chunk scatters with global atomics directly.
+//  * The contact adjoint reduces body-pose gradients across the wavefront before one
+//    atomic per body per wave.
+#include <limits.h>
+#include "mpm_device.h"
+#include "../../include/dexdeform_mpm.h"
+
+using namespace ddm;
+
+namespace {
+
+constexpr int kWG = 256;
+constexpr int kTileMax = 1024;   // LDS tile capacity in nodes (16 KB for 4 floats)
+constexpr int kTapeMax = 8;      // contact tape entries kept per node in the adjoint
+
+__device__ __forceinline__ int node_index(const ddm_scene &S, int x, int y, int z) {
+  int nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
+  int blk = ((x >> 2) * nby + (y >> 2)) * nbz + (z >> 2);
+  return ((S.blk_begin + blk) << 6) | (((x & 3) << 4) | ((y & 3) << 2) | (z & 3));
+}
+__device__ __forceinline__ int block_of(const ddm_scene &S, int bx, int by, int bz) {
+  return S.blk_begin + (bx * (S.dim[1] >> 2) + by) * (S.dim[2] >> 2) + bz;
+}
+
+struct BlockedV {
+  const float4 *g;
+  const ddm_scene *S;
+  __device__ f3 operator()(int x, int y, int z) const {
+    float4 a = g[node_index(*S, x, y, z)];
+    return f3{a.x, a.y, a.z};
+  }
+};
+struct BlockedGatherDMMV {
+  const float4 *g;
+  const ddm_scene *S;
+  __device__ void operator()(int x, int y, int z, f3 &mv, float &m) const {
+    float4 a = g[node_index(*S, x, y, z)];
+    m = a.x;
+    mv = f3{a.y, a.z, a.w};
+  }
+};
+
+// ---------------------------------------------------------------- LDS tile helpers
+struct Tile {
+  int lo[3], t[3], n;
+  __device__ int local(int x, int y, int z) const { return ((x - lo[0]) * t[1] + (y - lo[1])) * t[2] + (z - lo[2]); }
+};
+
+// bounding box of the chunk's stencil bases -> tile covering all stencil nodes
+__device__ Tile chunk_tile(bool valid, const int base[3], int *s_box) {
+  if (threadIdx.x < 3) {
+    s_box[threadIdx.x] = INT_MAX;
+    s_box[3 + threadIdx.x] = INT_MIN;
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&s_box[d], base[d]);
+      atomicMax(&s_box[3 + d], base[d]);
+    }
+  }
+  __syncthreads();
+  Tile T;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    T.lo[d] = s_box[d];
+    T.t[d] = s_box[3 + d] - s_box[d] + 3;
+  }
+  T.n = T.t[0] * T.t[1] * T.t[2];
+  __syncthreads();
+  return T;
+}
+
+// mark every node block that intersects the tile
+__device__ void flag_tile_blocks(const ddm_scene &S, const Tile &T, uint32_t *flags) {
+  int b0[3], nb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b0[d] = T.lo[d] >> 2;
+    nb[d] = ((T.lo[d] + T.t[d] - 1) >> 2) - b0[d] + 1;
+  }
+  int tot = nb[0] * nb[1] * nb[2];
+  for (int i = threadIdx.x; i < tot; i += kWG) {
+    int bx = i / (nb[1] * nb[2]), by = (i / nb[2]) % nb[1], bz = i % nb[2];
+    flags[block_of(S, b0[0] + bx, b0[1] + by, b0[2] + bz)] = 1u;
+  }
+}
+// mark the (at most 8) blocks of one particle's stencil
+__device__ void flag_stencil_blocks(const ddm_scene &S, const int base[3], uint32_t *flags) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c)
+        flags[block_of(S, (base[0] + 2 * a) >> 2, (base[1] + 2 * b) >> 2, (base[2] + 2 * c) >> 2)] = 1u;
+}
+
+// ---------------------------------------------------------------- P2G (+ F advection, SVD, return map)
+// BWD=false: forward substep, writes next.F.
+// BWD=true : adjoint recompute, writes the F_tmp/U/V/sigma temporaries instead.
+template <bool BWD>
+__global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, ddm_state cur, ddm_state next) {
+  __shared__ float tm[kTileMax], tx[kTileMax], ty[kTileMax], tz[kTileMax];
+  __shared__ int s_box[6];
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  const bool valid = p < ch.z;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  P2GParticle q;
+  if (valid) {
+    m33 F = ld9(cur.F + 9 * (size_t)p), C = ld9(cur.C + 9 * (size_t)p);
+    m33 Ft = advect_F(F, C, S.dt);
+    m33 U, V;
+    f3 sig;
+    svd3(Ft, U, sig, V);
+    m33 Fn;
+    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, B.mass[p], B.vol[p],
+                 ld3(B.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
+    if (BWD) {
+      st9(B.tmp_Ft + 9 * (size_t)p, Ft);
+      st9(B.tmp_U + 9 * (size_t)p, U);
+      st9(B.tmp_V + 9 * (size_t)p, V);
+      st3(B.tmp_sig + 3 * (size_t)p, sig);
+    } else {
+      st9(next.F + 9 * (size_t)p, Fn);
+    }
+  } else {
+    q.st.base[0] = q.st.base[1] = q.st.base[2] = 0;
+  }
+  Tile T = chunk_tile(valid, q.st.base, s_box);
+  float *gmmv = B.grid_mmv;
+  if (T.n <= kTileMax) {
+    for (int i = threadIdx.x; i < T.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            float cm;
+            f3 cmv;
+            p2g_contrib(q, S.dx, i, j, k, cm, cmv);
+            int li = T.local(q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
+            atomicAdd(&tm[li], cm);
+            atomicAdd(&tx[li], cmv.x);
+            atomicAdd(&ty[li], cmv.y);
+            atomicAdd(&tz[li], cmv.z);
+          }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.n; i += kWG) {
+      float a = tm[i], b = tx[i], c = ty[i], d = tz[i];
+      if (a != 0.f || b != 0.f || c != 0.f || d != 0.f) {
+        int x = T.lo[0] + i / (T.t[1] * T.t[2]), y = T.lo[1] + (i / T.t[2]) % T.t[1], z = T.lo[2] + i % T.t[2];
+        float *g = gmmv + 4 * (size_t)node_index(S, x, y, z);
+        atomicAdd(g + 0, a);
+        atomicAdd(g + 1, b);
+        atomicAdd(g + 2, c);
+        atomicAdd(g + 3, d);
+      }
+    }
+    flag_tile_blocks(S, T, B.block_flags);
+  } else if (valid) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float cm;
+          f3 cmv;
+          p2g_contrib(q, S.dx, i, j, k, cm, cmv);
+          float *g = gmmv + 4 * (size_t)node_index(S, q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
+          atomicAdd(g + 0, cm);
+          atomicAdd(g + 1, cmv.x);
+          atomicAdd(g + 2, cmv.y);
+          atomicAdd(g + 3, cmv.z);
+        }
+    flag_stencil_blocks(S, q.st.base, B.block_flags);
+  }
+}
+
+// ---------------------------------------------------------------- grid helpers
+__device__ __forceinline__ int scene_of_block(const ddm_scene *sc, int n, int blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (sc[mid].blk_begin <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+struct BlockCoord {
+  int ix, iy, iz;
+};
+__device__ __forceinline__ BlockCoord block_node(const ddm_scene &S, int blk, int lane) {
+  int lb = blk - S.blk_begin;
+  int nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
+  int bx = lb / (nby * nbz), by = (lb / nbz) % nby, bz = lb % nbz;
+  return BlockCoord{bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
+}
+__device__ __forceinline__ NodeParams scene_node_params(const ddm_batch &B, const ddm_scene &S, const ddm_state &cur,
+                                                        const ddm_state &next) {
+  NodeParams P;
+  P.dim[0] = S.dim[0];
+  P.dim[1] = S.dim[1];
+  P.dim[2] = S.dim[2];
+  P.nb = S.n_bodies;
+  P.dx = S.dx;
+  P.dt = S.dt;
+  P.gf = S.ground_friction;
+  P.gravity = f3{S.gravity[0], S.gravity[1], S.gravity[2]};
+  P.prim = B.body_prim + 4 * S.b_begin;
+  P.args = B.body_args + 4 * S.b_begin;
+  P.pos = cur.body_pos + 3 * S.b_begin;
+  P.rot = cur.body_rot + 4 * S.b_begin;
+  P.npos = next.body_pos + 3 * S.b_begin;
+  P.nrot = next.body_rot + 4 * S.b_begin;
+  return P;
+}
+
+// ---------------------------------------------------------------- grid update
+// ZERO=true (forward substep): consumes (m, mv) and clears them and the flags.
+template <bool ZERO>
+__global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, ddm_state cur, ddm_state next) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
+  float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
+  for (int blk = blockIdx.x * (kWG / 64) + wave; blk < B.n_blocks; blk += gridDim.x * (kWG / 64)) {
+    if (B.block_flags[blk] == 0u) continue;
+    const ddm_scene &S = B.scenes[scene_of_block(B.scenes, B.n_scenes, blk)];
+    BlockCoord c = block_node(S, blk, lane);
+    size_t gi = ((size_t)blk << 6) | (size_t)lane;
+    float4 a = mmv[gi];
+    f3 v{0.f, 0.f, 0.f};
+    if (node_active(a.x)) {
+      NodeParams P = scene_node_params(B, S, cur, next);
+      v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P);
+    }
+    gv[gi] = make_float4(v.x, v.y, v.z, 0.f);
+    if (ZERO) {
+      mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane == 0) B.block_flags[blk] = 0u;
+    }
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// adjoint of the grid update; clears (m, mv), the velocity adjoint and the flags
+__global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_state cur, ddm_state next, ddm_state cur_grad,
+                                                  ddm_state next_grad) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
+  float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
+  float4 *dmmv = reinterpret_cast<float4 *>(B.grid_dmmv);
+  for (int blk = blockIdx.x * (kWG / 64) + wave; blk < B.n_blocks; blk += gridDim.x * (kWG / 64)) {
+    if (B.block_flags[blk] == 0u) continue;
+    const ddm_scene &S = B.scenes[scene_of_block(B.scenes, B.n_scenes, blk)];
+    BlockCoord c = block_node(S, blk, lane);
+    size_t gi = ((size_t)blk << 6) | (size_t)lane;
+    float4 a = mmv[gi];
+    float4 d = dv[gi];
+    const float m = a.x;
+    const f3 mv{a.y, a.z, a.w};
+    const bool act = node_active(m);
+    NodeParams P = scene_node_params(B, S, cur, next);
+    const f3 gx = node_position(c.ix, c.iy, c.iz, P.dx);
+    const int mode = S.adjoint_mode;
+    // forward replay keeping the velocities entering each contact (integrator.cu:678-732)
+    f3 tape_v[kTapeMax];
+    int tape_b[kTapeMax];
+    int nt = 0;
+    f3 v0{0.f, 0.f, 0.f}, g{0.f, 0.f, 0.f};
+    if (act) {
+      f3 v = mv * (1.f / m);
+      v += P.dt * P.gravity;
+      v0 = v;
+      for (int b = 0; b < P.nb; ++b) {
+        f3 vin = v;
+        if (contact(v, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b), P.dt) && nt < kTapeMax) {
+          tape_v[nt] = vin;
+          tape_b[nt] = b;
+          ++nt;
+        }
+      }
+      g = boundary_adj(v, f3{d.x, d.y, d.z}, c.ix, c.iy, c.iz, P, mode);
+    }
+    // bodies in reverse order, uniformly across the wave so pose gradients can be reduced
+    for (int b = P.nb - 1; b >= 0; --b) {
+      PoseGrad pg = posegrad_zero();
+      bool mine = false;
+      if (act) {
+        Body Bd = load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b);
+        float infl;
+        if (in_contact(prim_sdf(Bd.prim, Bd.args, to_body(Bd.pos, Bd.rot, gx)), Bd.prim[2], infl)) {
+          mine = true;
+          f3 vin;
+          if (nt > 0 && tape_b[nt - 1] == b) {
+            vin = tape_v[--nt];
+          } else {  // tape overflow: replay the contacts before b
+            vin = v0;
+            for (int b2 = 0; b2 < b; ++b2)
+              contact(vin, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b2), P.dt);
+          }
+          contact_adj(vin, gx, Bd, P.dt, mode, g, pg);
+        }
+      }
+      if (__any(mine)) {
+        float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
+                          pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
+#pragma unroll
+        for (int e = 0; e < 14; ++e) vals[e] = wave_sum(vals[e]);
+        if (lane == 0) {
+          int gb = S.b_begin + b;
+          float *cp = cur_grad.body_pos + 3 * gb, *cr = cur_grad.body_rot + 4 * gb;
+          float *np = next_grad.body_pos + 3 * gb, *nr = next_grad.body_rot + 4 * gb;
+          atomicAdd(cp + 0, vals[0]);
+          atomicAdd(cp + 1, vals[1]);
+          atomicAdd(cp + 2, vals[2]);
+          atomicAdd(cr + 0, vals[3]);
+          atomicAdd(cr + 1, vals[4]);
+          atomicAdd(cr + 2, vals[5]);
+          atomicAdd(cr + 3, vals[6]);
+          atomicAdd(np + 0, vals[7]);
+          atomicAdd(np + 1, vals[8]);
+          atomicAdd(np + 2, vals[9]);
+          atomicAdd(nr + 0, vals[10]);
+          atomicAdd(nr + 1, vals[11]);
+          atomicAdd(nr + 2, vals[12]);
+          atomicAdd(nr + 3, vals[13]);
+        }
+      }
+    }
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+      f3 gmv = (float)(1.0 / (double)m) * g;  // integrator.cu:1054-1055
+      out = make_float4((-1.f / m / m) * dot3(mv, g), gmv.x, gmv.y, gmv.z);
+    }
+    dmmv[gi] = out;
+    mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+    dv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane == 0) B.block_flags[blk] = 0u;
+  }
+}
+
+// ---------------------------------------------------------------- G2P
+__global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, ddm_state cur, ddm_state next) {
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  if (p >= ch.z) return;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
+  f3 xo, vo;
+  m33 Co;
+  g2p_particle(ld3(cur.x + 3 * (size_t)p), gv, S.inv_dx, S.dx, S.dt, S.ground_height, dim, xo, vo, Co);
+  st3(next.x + 3 * (size_t)p, xo);
+  st3(next.v + 3 * (size_t)p, vo);
+  st9(next.C + 9 * (size_t)p, Co);
+}
+
+struct TileScatter3 {
+  float *tx, *ty, *tz;
+  const Tile *T;
+  __device__ void operator()(int x, int y, int z, f3 g) const {
+    int li = T->local(x, y, z);
+    atomicAdd(&tx[li], g.x);
+    atomicAdd(&ty[li], g.y);
+    atomicAdd(&tz[li], g.z);
+  }
+};
+struct GlobalScatter3 {
+  float *g;
+  const ddm_scene *S;
+  __device__ void operator()(int x, int y, int z, f3 v) const {
+    float *p = g + 4 * (size_t)node_index(*S, x, y, z);
+    atomicAdd(p + 0, v.x);
+    atomicAdd(p + 1, v.y);
+    atomicAdd(p + 2, v.z);
+  }
+};
+
+// adjoint of G2P: scatters the node-velocity adjoint, adds to the x adjoint
+__global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, ddm_state cur, ddm_state next, ddm_state cur_grad,
+                                                 ddm_state next_grad) {
+  __shared__ float tx[kTileMax], ty[kTileMax], tz[kTileMax];
+  __shared__ int s_box[6];
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  const bool valid = p < ch.z;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  f3 x{0.f, 0.f, 0.f};
+  Stencil st;
+  st.base[0] = st.base[1] = st.base[2] = 0;
+  if (valid) {
+    x = ld3(cur.x + 3 * (size_t)p);
+    stencil(x, S.inv_dx, dim, st);
+  }
+  Tile T = chunk_tile(valid, st.base, s_box);
+  BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
+  if (T.n <= kTileMax) {
+    for (int i = threadIdx.x; i < T.n; i += kWG) tx[i] = ty[i] = tz[i] = 0.f;
+    __syncthreads();
+    if (valid) {
+      TileScatter3 sc{tx, ty, tz, &T};
+      f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
+                                ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc, S.inv_dx,
+                                S.dx, S.dt, S.ground_height, dim);
+      add3(cur_grad.x + 3 * (size_t)p, gxp);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.n; i += kWG) {
+      float a = tx[i], b = ty[i], c = tz[i];
+      if (a != 0.f || b != 0.f || c != 0.f) {
+        int xx = T.lo[0] + i / (T.t[1] * T.t[2]), yy = T.lo[1] + (i / T.t[2]) % T.t[1], zz = T.lo[2] + i % T.t[2];
+        float *g = B.grid_dv + 4 * (size_t)node_index(S, xx, yy, zz);
+        atomicAdd(g + 0, a);
+        atomicAdd(g + 1, b);
+        atomicAdd(g + 2, c);
+      }
+    }
+  } else if (valid) {
+    GlobalScatter3 sc{B.grid_dv, &S};
+    f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
+                              ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc, S.inv_dx,
+                              S.dx, S.dt, S.ground_height, dim);
+    add3(cur_grad.x + 3 * (size_t)p, gxp);
+  }
+}
+
+// adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
+__global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, ddm_state cur, ddm_state cur_grad,
+                                                 ddm_state next_grad) {
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  if (p >= ch.z) return;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  const size_t p3 = 3 * (size_t)p, p9 = 9 * (size_t)p;
+  BlockedGatherDMMV gat{reinterpret_cast<const float4 *>(B.grid_dmmv), &S};
+  m33 F = ld9(cur.F + p9), C = ld9(cur.C + p9);
+  m33 Ft = ld9(B.tmp_Ft + p9), U = ld9(B.tmp_U + p9), V = ld9(B.tmp_V + p9);
+  f3 sig = ld3(B.tmp_sig + p3);
+  P2GAdj a;
+  p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, B.mass[p], B.vol[p],
+                   ld3(B.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
+  m33 gFt, gF, gC;
+  svd_adj(F, C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
+  add3(cur_grad.x + p3, a.x);
+  add3(cur_grad.v + p3, a.v);
+  m33 cg = ld9(cur_grad.C + p9);
+  cg = madd(madd(cg, a.C), gC);  // p2g_grad adds first, then compute_svd_grad (simulator.py:584-585)
+  st9(cur_grad.C + p9, cg);
+  add9(cur_grad.F + p9, gF);
+}
+
+__global__ void k_zero(float4 *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__global__ void k_zero_u32(uint32_t *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+inline int grid_wgs(const ddm_batch *b) {
+  int w = (b->n_blocks + (kWG / 64) - 1) / (kWG / 64);
+  return w < 4096 ? (w > 0 ? w : 1) : 4096;
+}
+inline int check(const ddm_batch *b, const ddm_state *s) {
+  if (!b || !s || b->n_chunks < 0 || b->n_blocks < 0 || b->n_scenes <= 0) return DDM_EINVAL;
+  if (b->max_bodies > DDM_MAX_BODIES) return DDM_ETOOMANYBODIES;
+  if (!b->scenes || !b->chunks || !b->grid_mmv || !b->grid_v || !b->block_flags) return DDM_EINVAL;
+  return 0;
+}
+
+}  // namespace
+
+#define DDM_API extern "C" __attribute__((visibility("default")))
+
+static inline void mark(hipEvent_t *ev, int i, hipStream_t stream) {
+  if (ev) (void)hipEventRecord(ev[i], stream);
+}
+
+static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream,
+                          hipEvent_t *ev) {
+  int e = check(b, cur);
+  if (e) return e;
+  if (!next) return DDM_EINVAL;
+  if (b->n_chunks == 0) return 0;
+  mark(ev, 0, stream);
+  k_p2g<false><<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next);
+  mark(ev, 1, stream);
+  k_grid<true><<<grid_wgs(b), kWG, 0, stream>>>(*b, *cur, *next);
+  mark(ev, 2, stream);
+  k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next);
+  mark(ev, 3, stream);
+  return (int)hipGetLastError();
+}
+
+static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
+                           const ddm_state *next_grad, hipStream_t stream, hipEvent_t *ev) {
+  int e = check(b, cur);
+  if (e) return e;
+  if (!next || !cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv || !b->tmp_Ft || !b->tmp_U || !b->tmp_V ||
+      !b->tmp_sig)
+    return DDM_EINVAL;
+  if (b->n_chunks == 0) return 0;
+  mark(ev, 0, stream);
+  k_p2g<true><<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next);
+  mark(ev, 1, stream);
+  k_grid<false><<<grid_wgs(b), kWG, 0, stream>>>(*b, *cur, *next);
+  mark(ev, 2, stream);
+  k_g2p_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next, *cur_grad, *next_grad);
+  mark(ev, 3, stream);
+  k_grid_adj<<<grid_wgs(b), kWG, 0, stream>>>(*b, *cur, *next, *cur_grad, *next_grad);
+  mark(ev, 4, stream);
+  k_p2g_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *cur_grad, *next_grad);
+  mark(ev, 5, stream);
+  return (int)hipGetLastError();
+}
+
+DDM_API int ddm_substep_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream) {
+  return launch_forward(b, cur, next, stream, nullptr);
+}
+
+DDM_API int ddm_substep_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                 const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
+  return launch_backward(b, cur, next, cur_grad, next_grad, stream, nullptr);
+}
+
+DDM_API int ddm_substep_forward_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                      hipStream_t stream, hipEvent_t *events) {
+  if (!events) return DDM_EINVAL;
+  return launch_forward(b, cur, next, stream, events);
+}
+
+DDM_API int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                       const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
+                                       hipEvent_t *events) {
+  if (!events) return DDM_EINVAL;
+  return launch_backward(b, cur, next, cur_grad, next_grad, stream, events);
+}
+
+DDM_API int ddm_batch_reset(const ddm_batch *b, hipStream_t stream) {
+  if (!b || b->n_blocks < 0) return DDM_EINVAL;
+  size_t nodes = (size_t)b->n_blocks * 64;
+  if (nodes == 0) return 0;
+  int wg = 1024;
+  k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_mmv), nodes);
+  k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_v), nodes);
+  if (b->grid_dv) k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_dv), nodes);
+  if (b->grid_dmmv) k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_dmmv), nodes);
+  k_zero_u32<<<wg, kWG, 0, stream>>>(b->block_flags, (size_t)b->n_blocks);
+  return (int)hipGetLastError();
+}

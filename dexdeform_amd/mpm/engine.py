@@ -1,0 +1,260 @@
+"""Device-resident engine for one or many MLS-MPM scenes.
+
+An :class:`Engine` owns everything the fused kernels need for a fixed set of scenes:
+per-particle material arrays, primitive parameters, the block-sparse grid work arrays
+(4x4x4-blocked, float4 per node) and the scene/chunk tables. States are
+:class:`StateBuffers` (particle x, v, F, C and body poses) allocated by the caller, so
+a single-scene simulator, a batched rollout and the rollout server all drive the same
+two entry points, ``ddm_substep_forward`` and ``ddm_substep_backward``.
+
+The forward substep replaces the reference's ``compute_svd -> p2g -> grid_op_v2 -> g2p``
+sequence (mpm/simulator.py:561-571); the backward replaces
+``substep_grad`` (mpm/simulator.py:573-585) with the same recompute-then-adjoint order.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+CHUNK = 256  # particles per workgroup in the particle kernels
+
+
+@dataclass
+class SceneSpec:
+    """Constants of one scene, in kernel units.
+
+    gravity is the acceleration the grid update adds (the reference uploads the configured
+    gravity multiplied by 30, mpm/simulator.py:385). body_prim rows are
+    (type, friction, softness, round) and body_args rows the shape arguments
+    (mpm/simulator.py:396-398, mpm/cuda_env.py:80-101).
+    """
+
+    n_particles: int
+    grid_dim: Sequence[int]
+    dx: float
+    dt: float
+    gravity: Sequence[float]
+    mass: np.ndarray
+    vol: np.ndarray
+    mu_lam_yield: np.ndarray
+    body_prim: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    body_args: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    ground_friction: float = 0.0
+    ground_height: float = 3.0
+    adjoint_mode: int = 0  # 0 reference adjoint, 1 exact
+
+    @property
+    def n_bodies(self) -> int:
+        return int(len(self.body_prim))
+
+    def validate(self):
+        gd = tuple(int(g) for g in self.grid_dim)
+        if len(gd) != 3 or any(g < 8 or g % 4 for g in gd):
+            raise ValueError(f"grid dims must be multiples of 4 and >= 8, got {gd}")
+        if self.n_bodies > _lib.DDM_MAX_BODIES:
+            raise ValueError(f"at most {_lib.DDM_MAX_BODIES} bodies per scene, got {self.n_bodies}")
+        for name in ("mass", "vol"):
+            if np.asarray(getattr(self, name)).shape != (self.n_particles,):
+                raise ValueError(f"{name} must have shape ({self.n_particles},)")
+        if np.asarray(self.mu_lam_yield).shape != (self.n_particles, 3):
+            raise ValueError("mu_lam_yield must have shape (n_particles, 3)")
+
+
+def lame(E: float, nu: float):
+    """(mu, lambda) as MPMSimulator.__init__ computes them (mpm/simulator.py:211-212)."""
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+class StateBuffers:
+    """Particle state (x, v, F, C) of every scene of an engine plus body poses."""
+
+    def __init__(self, n_particles: int, n_bodies: int, device, identity_F: bool = True):
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=device)
+        self.x = z(n_particles, 3)
+        self.v = z(n_particles, 3)
+        self.F = z(n_particles, 3, 3)
+        if identity_F and n_particles:
+            self.F[:] = torch.eye(3, device=device)
+        self.C = z(n_particles, 3, 3)
+        self.body_pos = z(max(n_bodies, 0), 3)
+        self.body_rot = z(max(n_bodies, 0), 4)
+        if n_bodies:
+            self.body_rot[:, 0] = 1.0
+        self._ct = None
+
+    def tensors(self):
+        return (self.x, self.v, self.F, self.C, self.body_pos, self.body_rot)
+
+    def ctype(self) -> _lib.DdmState:
+        if self._ct is None:
+            self._ct = _lib.DdmState(*(_lib.ptr(t) for t in self.tensors()))
+        return self._ct
+
+    def zero_(self):
+        for t in self.tensors():
+            t.zero_()
+        return self
+
+    def copy_(self, other: "StateBuffers"):
+        for a, b in zip(self.tensors(), other.tensors()):
+            a.copy_(b)
+        return self
+
+
+def chunk_table(p_begins: Sequence[int], counts: Sequence[int]) -> np.ndarray:
+    """(scene, begin, end, 0) rows covering each scene's particles in <= CHUNK pieces."""
+    rows = []
+    for s, (b, n) in enumerate(zip(p_begins, counts)):
+        for c in range(0, int(n), CHUNK):
+            rows.append((s, b + c, b + min(c + CHUNK, int(n)), 0))
+    return np.asarray(rows, dtype=np.int32).reshape(-1, 4)
+
+
+class Engine:
+    def __init__(self, scenes: List[SceneSpec], device="cuda:0", with_grad: bool = True):
+        if not scenes:
+            raise ValueError("an engine needs at least one scene")
+        for s in scenes:
+            s.validate()
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        self.scenes = scenes
+        self.with_grad = with_grad
+        caps = [int(s.n_particles) for s in scenes]
+        self.p_begin = np.concatenate([[0], np.cumsum(caps)[:-1]]).astype(np.int64)
+        self.capacity = caps
+        self.n_particles_total = int(sum(caps))
+        nbs = [s.n_bodies for s in scenes]
+        self.b_begin = np.concatenate([[0], np.cumsum(nbs)[:-1]]).astype(np.int64)
+        self.n_bodies_total = int(sum(nbs))
+        nblk = [(int(s.grid_dim[0]) // 4) * (int(s.grid_dim[1]) // 4) * (int(s.grid_dim[2]) // 4) for s in scenes]
+        self.blk_begin = np.concatenate([[0], np.cumsum(nblk)[:-1]]).astype(np.int64)
+        self.n_blocks = int(sum(nblk))
+        if self.n_blocks * 64 >= 2 ** 31 or self.n_particles_total * 9 >= 2 ** 31:
+            raise ValueError("batch too large for 32-bit node/particle indices")
+
+        dev = self.device
+        f32 = lambda a, shape: torch.as_tensor(np.asarray(a, np.float32).reshape(shape), device=dev)
+        P, NB = self.n_particles_total, self.n_bodies_total
+        self.mass = f32(np.concatenate([np.asarray(s.mass, np.float32) for s in scenes]), (P,))
+        self.vol = f32(np.concatenate([np.asarray(s.vol, np.float32) for s in scenes]), (P,))
+        self.mu_lam_yield = f32(np.concatenate([np.asarray(s.mu_lam_yield, np.float32).reshape(-1, 3)
+                                                for s in scenes]), (P, 3))
+        prim = [np.asarray(s.body_prim, np.float32).reshape(-1, 4) for s in scenes]
+        args = [np.asarray(s.body_args, np.float32).reshape(-1, 4) for s in scenes]
+        self.body_prim = f32(np.concatenate(prim) if NB else np.zeros((0, 4)), (NB, 4))
+        self.body_args = f32(np.concatenate(args) if NB else np.zeros((0, 4)), (NB, 4))
+
+        G = self.n_blocks * 64
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
+        self.grid_mmv = z(G, 4)
+        self.grid_v = z(G, 4)
+        self.block_flags = torch.zeros(self.n_blocks, dtype=torch.int32, device=dev)
+        if with_grad:
+            self.grid_dv = z(G, 4)
+            self.grid_dmmv = z(G, 4)
+            self.tmp_Ft = z(P, 9)
+            self.tmp_U = z(P, 9)
+            self.tmp_V = z(P, 9)
+            self.tmp_sig = z(P, 3)
+        else:
+            self.grid_dv = self.grid_dmmv = self.tmp_Ft = self.tmp_U = self.tmp_V = self.tmp_sig = None
+
+        tab = np.zeros(len(scenes), dtype=_lib.SCENE_DTYPE)
+        for i, s in enumerate(scenes):
+            tab[i]["p_begin"] = self.p_begin[i]
+            tab[i]["p_count"] = s.n_particles
+            tab[i]["blk_begin"] = self.blk_begin[i]
+            tab[i]["n_blocks"] = nblk[i]
+            tab[i]["dim"] = [int(g) for g in s.grid_dim]
+            tab[i]["n_bodies"] = s.n_bodies
+            tab[i]["b_begin"] = self.b_begin[i]
+            tab[i]["adjoint_mode"] = int(s.adjoint_mode)
+            tab[i]["dx"] = s.dx
+            tab[i]["inv_dx"] = 1.0 / float(s.dx)
+            tab[i]["dt"] = s.dt
+            tab[i]["ground_friction"] = s.ground_friction
+            tab[i]["ground_height"] = s.ground_height
+            tab[i]["gravity"] = np.asarray(s.gravity, np.float32)
+        self._scene_np = tab
+        self.scene_table = torch.as_tensor(tab.view(np.uint8).copy(), device=dev)
+        self.counts = list(caps)
+        self.chunks = None
+        self._set_chunks(self.counts)
+        self._batch = None
+        self._rebuild_batch()
+
+    # ------------------------------------------------------------------ tables
+    def _set_chunks(self, counts):
+        tab = chunk_table(self.p_begin, counts)
+        self.n_chunks = int(len(tab))
+        self.chunks = torch.as_tensor(tab if len(tab) else np.zeros((1, 4), np.int32), device=self.device)
+
+    def set_counts(self, counts: Sequence[int]):
+        """Number of live particles per scene (<= capacity); the rest are ignored."""
+        counts = [int(c) for c in counts]
+        for c, cap in zip(counts, self.capacity):
+            if c < 0 or c > cap:
+                raise ValueError(f"particle count {c} outside [0, {cap}]")
+        if counts != self.counts:
+            self.counts = counts
+            self._set_chunks(counts)
+            self._rebuild_batch()
+
+    def _rebuild_batch(self):
+        p = _lib.ptr
+        self._batch = _lib.DdmBatch(
+            len(self.scenes), self.n_chunks, self.n_blocks, max(s.n_bodies for s in self.scenes),
+            p(self.scene_table), p(self.chunks), p(self.mass), p(self.vol), p(self.mu_lam_yield),
+            p(self.body_prim) if self.n_bodies_total else None, p(self.body_args) if self.n_bodies_total else None,
+            p(self.grid_mmv), p(self.grid_v), p(self.grid_dv), p(self.grid_dmmv), p(self.block_flags),
+            p(self.tmp_Ft), p(self.tmp_U), p(self.tmp_V), p(self.tmp_sig))
+
+    def update_body_params(self, body_prim: torch.Tensor = None, body_args: torch.Tensor = None):
+        if body_prim is not None:
+            self.body_prim.copy_(torch.as_tensor(body_prim, dtype=torch.float32).reshape(self.body_prim.shape))
+        if body_args is not None:
+            self.body_args.copy_(torch.as_tensor(body_args, dtype=torch.float32).reshape(self.body_args.shape))
+
+    # ------------------------------------------------------------------ states
+    def new_state(self) -> StateBuffers:
+        return StateBuffers(self.n_particles_total, self.n_bodies_total, self.device)
+
+    def new_grad(self) -> StateBuffers:
+        return StateBuffers(self.n_particles_total, self.n_bodies_total, self.device, identity_F=False)
+
+    def scene_slice(self, i: int, n: Optional[int] = None) -> slice:
+        b = int(self.p_begin[i])
+        return slice(b, b + (self.counts[i] if n is None else n))
+
+    def body_slice(self, i: int) -> slice:
+        b = int(self.b_begin[i])
+        return slice(b, b + self.scenes[i].n_bodies)
+
+    # ------------------------------------------------------------------ substeps
+    def forward(self, cur: StateBuffers, nxt: StateBuffers, stream=None):
+        st = _lib.stream_ptr(stream)
+        _lib.check(self.lib.ddm_substep_forward(ctypes.byref(self._batch), ctypes.byref(cur.ctype()),
+                                                ctypes.byref(nxt.ctype()), st), "ddm_substep_forward")
+
+    def backward(self, cur: StateBuffers, nxt: StateBuffers, gcur: StateBuffers, gnxt: StateBuffers, stream=None):
+        if not self.with_grad:
+            raise RuntimeError("engine was created with with_grad=False")
+        st = _lib.stream_ptr(stream)
+        _lib.check(self.lib.ddm_substep_backward(ctypes.byref(self._batch), ctypes.byref(cur.ctype()),
+                                                 ctypes.byref(nxt.ctype()), ctypes.byref(gcur.ctype()),
+                                                 ctypes.byref(gnxt.ctype()), st), "ddm_substep_backward")
+
+    def reset_grid(self, stream=None):
+        _lib.check(self.lib.ddm_batch_reset(ctypes.byref(self._batch), _lib.stream_ptr(stream)), "ddm_batch_reset")
+
+    def nbytes(self) -> int:
+        ts = [self.mass, self.vol, self.mu_lam_yield, self.body_prim, self.body_args, self.grid_mmv, self.grid_v,
+              self.block_flags, self.grid_dv, self.grid_dmmv, self.tmp_Ft, self.tmp_U, self.tmp_V, self.tmp_sig]
+        return int(sum(t.numel() * t.element_size() for t in ts if t is not None))

@@ -1,0 +1,184 @@
+"""Device-resident differentiable rollouts with substep checkpointing.
+
+A :class:`Rollout` runs ``T`` forward substeps of every scene of an :class:`Engine`
+and then the adjoint substeps in reverse, producing the gradients of a loss on the
+final state with respect to every substep's body poses (and the initial state).
+
+Memory: with ``checkpoint_every = K`` only every K-th state is kept during the
+forward pass (the reference keeps all of them, mpm/simulator.py:205-206); the backward
+pass recomputes one K-substep segment at a time from its checkpoint into a K+1 state
+buffer and runs the adjoint through it. States then cost (T/K + K + 1) instead of T+1
+copies, at the price of one extra forward substep per substep.
+
+Body poses of all substeps live in ``pose_pos [T+1, NB, 3]`` / ``pose_rot [T+1, NB, 4]``;
+the states' pose pointers alias those rows, so setting a trajectory's poses is one
+device copy and nothing crosses the host per substep. Pose gradients accumulate into
+``pos_grad`` / ``rot_grad`` with the same shapes.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional
+
+import torch
+
+from .. import _lib
+from .engine import Engine
+
+
+class _View:
+    """A DdmState whose particle arrays and pose rows come from different tensors."""
+
+    __slots__ = ("x", "v", "F", "C", "body_pos", "body_rot", "_ct")
+
+    def __init__(self, x, v, F, C, body_pos, body_rot):
+        self.x, self.v, self.F, self.C, self.body_pos, self.body_rot = x, v, F, C, body_pos, body_rot
+        self._ct = _lib.DdmState(*(_lib.ptr(t) for t in (x, v, F, C, body_pos, body_rot)))
+
+    def ctype(self):
+        return self._ct
+
+
+class _Particles:
+    def __init__(self, n, device, identity=True):
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=device)
+        self.x, self.v, self.F, self.C = z(n, 3), z(n, 3), z(n, 3, 3), z(n, 3, 3)
+        if identity and n:
+            self.F[:] = torch.eye(3, device=device)
+
+    def tensors(self):
+        return (self.x, self.v, self.F, self.C)
+
+    def copy_(self, o):
+        for a, b in zip(self.tensors(), o.tensors()):
+            a.copy_(b)
+
+    def zero_(self):
+        for a in self.tensors():
+            a.zero_()
+
+
+class Rollout:
+    def __init__(self, engine: Engine, T: int, checkpoint_every: Optional[int] = None):
+        self.engine = engine
+        self.T = int(T)
+        dev = engine.device
+        P, NB = engine.n_particles_total, engine.n_bodies_total
+        self.K = int(checkpoint_every) if checkpoint_every else self.T
+        if self.K <= 0 or self.T % self.K:
+            raise ValueError("checkpoint_every must divide the horizon")
+        self.full = self.K == self.T
+        nb_rows = max(NB, 1)
+        self.pose_pos = torch.zeros(self.T + 1, nb_rows, 3, device=dev)
+        self.pose_rot = torch.zeros(self.T + 1, nb_rows, 4, device=dev)
+        self.pose_rot[..., 0] = 1.0
+        self.pos_grad = torch.zeros_like(self.pose_pos)
+        self.rot_grad = torch.zeros_like(self.pose_rot)
+        # checkpoints at t = 0, K, 2K, ... (all states when K == T)
+        n_ckpt = self.T // self.K + 1
+        self.ckpt = [_Particles(P, dev) for _ in range(n_ckpt)] if not self.full else []
+        # segment buffer: K+1 states (holds the whole trajectory when not checkpointing)
+        self.seg = [_Particles(P, dev) for _ in range(self.K + 1)]
+        self.g = [_Particles(P, dev, identity=False) for _ in range(2)]
+        pp, pr, gp, gr = self.pose_pos, self.pose_rot, self.pos_grad, self.rot_grad
+        # state view of segment slot j at absolute substep t
+        self._sv = {}
+        self._gv = {}
+
+        def sview(j, t):
+            key = (j, t)
+            if key not in self._sv:
+                s = self.seg[j]
+                self._sv[key] = _View(s.x, s.v, s.F, s.C, pp[t], pr[t])
+            return self._sv[key]
+
+        def gview(j, t):
+            key = (j, t)
+            if key not in self._gv:
+                s = self.g[j]
+                self._gv[key] = _View(s.x, s.v, s.F, s.C, gp[t], gr[t])
+            return self._gv[key]
+
+        self._sview, self._gview = sview, gview
+
+    # ------------------------------------------------------------------ inputs / outputs
+    def set_initial(self, x, v=None, F=None, C=None):
+        s0 = self.seg[0]
+        s0.x.copy_(x)
+        s0.v.copy_(v) if v is not None else s0.v.zero_()
+        if F is not None:
+            s0.F.copy_(F)
+        else:
+            s0.F[:] = torch.eye(3, device=s0.F.device)
+        s0.C.copy_(C) if C is not None else s0.C.zero_()
+        if not self.full:
+            self.ckpt[0].copy_(s0)
+
+    def set_poses(self, pos, rot):
+        """pos [T+1, NB, 3], rot [T+1, NB, 4] (row t = poses of substep t)."""
+        if self.engine.n_bodies_total:
+            self.pose_pos.copy_(pos)
+            self.pose_rot.copy_(rot)
+
+    def final_state(self) -> _Particles:
+        return self.seg[self.K] if self.full else self._final
+
+    # ------------------------------------------------------------------ passes
+    def _fwd(self, j, t):
+        eng = self.engine
+        _lib.check(eng.lib.ddm_substep_forward(ctypes.byref(eng._batch), ctypes.byref(self._sview(j, t).ctype()),
+                                               ctypes.byref(self._sview(j + 1, t + 1).ctype()),
+                                               _lib.stream_ptr()), "ddm_substep_forward")
+
+    def forward(self):
+        """T forward substeps from the initial state."""
+        if self.full:
+            for t in range(self.T):
+                self._fwd(t, t)
+            return self.seg[self.T]
+        # checkpointed: run each segment in the segment buffer and keep its end state
+        for c in range(self.T // self.K):
+            if c > 0:
+                self.seg[0].copy_(self.ckpt[c])
+            for j in range(self.K):
+                self._fwd(j, c * self.K + j)
+            self.ckpt[c + 1].copy_(self.seg[self.K])
+        self._final = self.ckpt[-1]
+        return self._final
+
+    def backward(self, x_grad=None, v_grad=None, F_grad=None, C_grad=None):
+        """Adjoint of the whole rollout for the given gradient of the final state.
+        Afterwards ``pos_grad``/``rot_grad`` hold d loss / d poses of every substep and
+        ``initial_grad()`` the gradient of the initial particle state."""
+        eng = self.engine
+        self.pos_grad.zero_()
+        self.rot_grad.zero_()
+        gn = self.g[0]
+        for a, b in zip(gn.tensors(), (x_grad, v_grad, F_grad, C_grad)):
+            a.copy_(b) if b is not None else a.zero_()
+        cur_slot = 0
+        n_seg = self.T // self.K
+        for c in range(n_seg - 1, -1, -1):
+            base = c * self.K
+            if not self.full:
+                self.seg[0].copy_(self.ckpt[c])
+                for j in range(self.K):
+                    self._fwd(j, base + j)
+            for j in range(self.K - 1, -1, -1):
+                t = base + j
+                nxt_slot = cur_slot
+                cur_slot = 1 - cur_slot
+                self.g[cur_slot].zero_()
+                _lib.check(eng.lib.ddm_substep_backward(
+                    ctypes.byref(eng._batch), ctypes.byref(self._sview(j, t).ctype()),
+                    ctypes.byref(self._sview(j + 1, t + 1).ctype()), ctypes.byref(self._gview(cur_slot, t).ctype()),
+                    ctypes.byref(self._gview(nxt_slot, t + 1).ctype()), _lib.stream_ptr()), "ddm_substep_backward")
+        self._g0 = self.g[cur_slot]
+        return self._g0
+
+    def initial_grad(self) -> _Particles:
+        return self._g0
+
+    def state_bytes(self) -> int:
+        per = self.engine.n_particles_total * (3 + 3 + 9 + 9) * 4
+        return per * (len(self.ckpt) + len(self.seg) + len(self.g))

@@ -1,0 +1,170 @@
+"""Autograd bridge over the simulator (mirrors mpm/torch_wrapper.py:7-184).
+
+``GradModel.forward(s, action, *past_obs)`` runs one environment step (``substeps``
+substeps) as one autograd node; ``loss.backward()`` runs the hand-written adjoint
+substeps in reverse and returns pose gradients that flow into the torch kinematics
+(``compute_forward_kinematics``) and from there to the actions.
+
+As in the reference, the particle-state adjoint is carried between environment steps
+inside the simulator's per-state gradient buffers, and the observation gradients of
+past observations are returned as zeros. Unlike the reference, observations and their
+gradients never leave the device (no numpy round trips).
+"""
+from __future__ import annotations
+
+import torch
+from torch.autograd import Function
+
+
+class GradModel:
+    def __init__(self, env, return_grid=(-1,), return_svd=False):
+        self.env = env
+        self.sim = env.simulator if hasattr(env, "simulator") else env
+        self.dim = 3
+        assert len(return_grid) == 0 or return_grid[0] == -1
+        self.primitives = list(range(self.sim.n_bodies))
+        self._forward_func = None
+        self._set_pose_func = None
+        self.return_grid = return_grid
+        self.return_svd = return_svd
+        self.device = self.sim.device
+        self.pos_rot = None
+
+    @property
+    def substeps(self):
+        return self.sim.substeps
+
+    def zero_grad(self, return_grid=None, return_svd=None, **kwargs):
+        self.sim.states[0].clear_grad()
+        if return_grid is not None:
+            self.return_grid = return_grid
+        if return_svd is not None:
+            self.return_svd = return_svd or self.return_svd
+
+    def wrap_obs(self, obs):
+        output = {"pos": obs[0][:, :3], "vel": obs[0][:, 3:6], "tool": obs[1], "dist": obs[0][:, 6:]}
+        obs = obs[2:]
+        if len(self.return_grid) > 0:
+            ng = len(self.return_grid)
+            grid, obs = obs[:ng], obs[ng:]
+            output["grid"] = {k: v for k, v in zip(self.return_grid, grid)}
+        assert len(obs) == 0
+        return output
+
+    def get_obs(self, s, device):
+        """(particles [N, 6+nb] = x|v|dists, tools [nb, 7], grid mass) at env step s
+        (mpm/torch_wrapper.py:54-77)."""
+        sim = self.sim
+        f = s * sim.substeps
+        st = sim.states[f]
+        n = sim.n_particles
+        x = st.x.tensor[:n].detach().clone()
+        v = st.v.tensor[:n].detach().clone()
+        c = torch.cat((st.body_pos.tensor, st.body_rot.tensor), 1).detach().clone()
+        outs = []
+        if sim.n_bodies:
+            dists = sim.get_dists(f, device=device)
+            outs.append(torch.cat((x, v, dists.to(x.device)), 1))
+        else:
+            outs.append(torch.cat((x, v), 1))
+        outs.append(c)
+        for i in self.return_grid:
+            outs.append(sim.compute_grid_mass(f, i, device=device))
+        return tuple(o.to(device) for o in outs)
+
+    def set_obs_grad(self, s, particle_grad, tool_grad, *args):
+        """Adds observation gradients into the state-s adjoint buffers
+        (mpm/torch_wrapper.py:79-105)."""
+        sim = self.sim
+        f = s * sim.substeps
+        for idx, i in enumerate(self.return_grid):
+            if args[idx] is not None:
+                sim.compute_grid_mass(f, i, backward_grad=args[idx])
+        nb = sim.n_bodies
+        st = sim.states[f]
+        n = sim.n_particles
+        if particle_grad is not None:
+            particle_grad = particle_grad.to(st.x.tensor.device, torch.float32)
+            if nb:
+                sim.get_dists(f, particle_grad[:, -nb:].contiguous())
+                particle_grad = particle_grad[:, :-nb]
+            pg = particle_grad.reshape(-1, self.dim * 2)
+            st.x_grad.tensor[:n] += pg[:, :3]
+            st.v_grad.tensor[:n] += pg[:, 3:]
+        if tool_grad is not None and nb:
+            tg = tool_grad.to(st.x.tensor.device, torch.float32).reshape(nb, 7)
+            st.body_pos_grad.tensor += tg[:, :3]
+            st.body_rot_grad.tensor += tg[:, 3:]
+
+    @property
+    def diff_forward(self):
+        if self._forward_func is None:
+            model = self
+
+            class Forward(Function):
+                @staticmethod
+                def forward(ctx, s, pos, rot, *past_obs):
+                    ctx.s = s
+                    ctx.save_for_backward(*[torch.zeros_like(i) for i in past_obs])
+                    sim = model.sim
+                    f = s * model.substeps
+                    for i in range(model.substeps):
+                        sim.set_pose(sim.states[f + i + 1], pos[i], rot[i])
+                        sim.substep(f + i, clear_grad=True)
+                    return model.get_obs(s + 1, pos.device)
+
+                @staticmethod
+                def backward(ctx, *obs_grad):
+                    zero_grads = ctx.saved_tensors
+                    s = ctx.s
+                    sim = model.sim
+                    model.set_obs_grad(s + 1, *obs_grad)
+                    f = s * model.substeps
+                    pos, rot = [], []
+                    for i in range(f + model.substeps - 1, f - 1, -1):
+                        sim.substep_grad(i)
+                        st = sim.states[i + 1]
+                        pos.append(st.body_pos_grad.tensor.clone())
+                        rot.append(st.body_rot_grad.tensor.clone())
+                    pos_grad = torch.stack(pos[::-1]).to(model.device)
+                    rot_grad = torch.stack(rot[::-1]).to(model.device)
+                    return (None, pos_grad, rot_grad) + tuple(zero_grads)
+
+            self._forward_func = Forward
+        return self._forward_func.apply
+
+    @property
+    def diff_set_pose(self):
+        if self._set_pose_func is None:
+            model = self
+
+            class SetPose(Function):
+                @staticmethod
+                def forward(ctx, s, pos, rot):
+                    ctx.s = s
+                    model.sim.set_pose(model.sim.states[s * model.substeps], pos, rot)
+                    return model.get_obs(s, pos.device)
+
+                @staticmethod
+                def backward(ctx, *obs_grad):
+                    s = ctx.s
+                    model.set_obs_grad(s, *obs_grad)
+                    st = model.sim.states[s * model.substeps]
+                    pos_grad = st.body_pos_grad.tensor.clone()
+                    rot_grad = st.body_rot_grad.tensor.clone()
+                    st.body_pos_grad.zero()
+                    st.body_rot_grad.zero()
+                    return (None, pos_grad, rot_grad)
+
+            self._set_pose_func = SetPose.apply
+        return self._set_pose_func
+
+    def forward(self, s, action, *past_obs, pos_rot=None):
+        if s == 0 and pos_rot is None:
+            self.pos_rot = self.sim.download_pos_rot(0, action.device)
+        if pos_rot is not None:
+            assert isinstance(pos_rot[0], torch.Tensor)
+            past_obs = self.diff_set_pose(s, *pos_rot)
+        pos, rot, q_state = self.sim.compute_forward_kinematics(s * self.substeps, action, pos_rot=self.pos_rot)
+        self.pos_rot = q_state
+        return self.diff_forward(s, pos, rot, *past_obs)

@@ -1,0 +1,175 @@
+/* dexdeform_mpm.h — C ABI of the MI355X (gfx950) MLS-MPM substep library
+ * libdexdeform_mpm.so.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Per-stage launchers with the names, argument order and meaning of the
+ *     reference's extern "C" launchers in mpm/csrc/integrator.cu (bound from Python by
+ *     ctypes in mpm/types.py:103-290). Pointers are device pointers to FP32 AoS arrays
+ *     (vec3 = 3 floats, mat3 = 9 floats row-major, quat = 4 floats (w,x,y,z)); an
+ *     `ivec3 const&` becomes `const int *` to 3 host ints (the same bits a ctypes
+ *     POINTER(ivec3) passes); `cudaStream_t` becomes hipStream_t. Unlike the reference
+ *     (which returns void and never checks launches, common.h:20-25) every function
+ *     returns 0 on success or a nonzero hipError_t / DDM_E* code.
+ *
+ *  2. Fused, multi-scene entry points (ddm_*) used by the Python simulator: one launch
+ *     per stage for all scenes of a batch, block-sparse grids in a 4x4x4 blocked
+ *     layout, LDS-tiled scatters. They have no single counterpart in the reference:
+ *     ddm_substep_forward replaces the sequence MPMSimulator.substep
+ *     (mpm/simulator.py:561-571) and ddm_substep_backward replaces
+ *     MPMSimulator.substep_grad (mpm/simulator.py:573-585), for every scene at once.
+ *
+ * All work is enqueued on the given stream; no function synchronises the device or
+ * allocates device memory, so the sequences can be captured in a HIP graph.
+ */
+#ifndef DEXDEFORM_MPM_H
+#define DEXDEFORM_MPM_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes beyond hipError_t */
+#define DDM_EINVAL 10001   /* bad argument (null pointer, size, dims not multiple of 4) */
+#define DDM_ETOOMANYBODIES 10002
+
+/* ---------------------------------------------------------------- group 1
+ * Reference-shaped launchers. n / dim = number of particles. */
+
+/* replaces compute_svd, integrator.cu:1627-1638 */
+int compute_svd(float *F, float *C, float *newF, float *U, float *V, float *sig, float dt, int dim,
+                hipStream_t stream);
+
+/* replaces compute_svd_grad, integrator.cu:1640-1657 */
+int compute_svd_grad(float *F, float *C, float *U, float *V, float *sig, float *newF_grad, float *U_grad,
+                     float *V_grad, float *sig_grad, float *F_grad, float *C_grad, float dt, int dim,
+                     hipStream_t stream);
+
+/* replaces p2g, integrator.cu:1659-1690 */
+int p2g(float *particle_x, float *particle_v, float *particle_m, float *particle_vol, float *particle_F,
+        float *particle_U, float *particle_sig, float *particle_V, float *particle_C, float *particle_mu_lam_yield,
+        int *grid_lower, const int *grid_dim, float dx, float inv_dx, float dt, float *out_particle_F,
+        float *out_grid_mv, float *out_grid_m, int dim, hipStream_t stream);
+
+/* replaces p2g_grad, integrator.cu:1692-1742 */
+int p2g_grad(float *particle_x, float *particle_v, float *particle_m, float *particle_vol, float *particle_F,
+             float *particle_U, float *particle_sig, float *particle_V, float *particle_C,
+             float *particle_mu_lam_yield, int *grid_lower, const int *grid_dim, float dx, float inv_dx, float dt,
+             float *out_particle_F, float *out_grid_mv, float *out_grid_m, float *particle_x_grad,
+             float *particle_v_grad, float *particle_F_grad, float *particle_C_grad, float *particle_U_grad,
+             float *particle_sig_grad, float *particle_V_grad, float *out_particle_F_grad, float *out_grid_v_grad,
+             float *out_grid_m_grad, int dim, hipStream_t stream);
+
+/* replaces grid_op_v2, integrator.cu:1754-1779 (grid_body_v_in may be NULL: no tape) */
+int grid_op_v2(float *grid_m, float *grid_v_in, float *grid_body_v_in, int *grid_lower, float *gravity,
+               float *body_pos, float *body_rot, float *next_body_pos, float *next_body_rot,
+               float *body_type_friction_softness_round, float *body_args, float dx, float inv_dx, float dt,
+               float ground_friction, float *out_grid_v, const int *grid_dim, int n_bodies, hipStream_t stream);
+
+/* replaces grid_op_v2_grad, integrator.cu:1781-1819 (reads the tape written by grid_op_v2) */
+int grid_op_v2_grad(float *grid_m, float *grid_v_in, float *grid_body_v_in, int *grid_lower, float *gravity,
+                    float *body_pos, float *body_rot, float *next_body_pos, float *next_body_rot,
+                    float *body_type_friction_softness_round, float *body_args, float *grid_m_grad,
+                    float *grid_v_in_grad, float *body_pos_grad, float *body_rot_grad, float *next_body_pos_grad,
+                    float *next_body_rot_grad, float dx, float inv_dx, float dt, float ground_friction,
+                    float *out_grid_v, float *out_grid_v_grad, const int *grid_dim, int n_bodies,
+                    hipStream_t stream);
+
+/* replaces g2p, integrator.cu:1821-1835 */
+int g2p(float *particle_x, float *grid_v, int *grid_lower, float dx, float inv_dx, float dt, const int *grid_dim,
+        float *out_particle_v, float ground_height, float *out_particle_C, float *out_particle_x, int dim,
+        hipStream_t stream);
+
+/* replaces g2p_grad, integrator.cu:1837-1861 */
+int g2p_grad(float *particle_x, float *grid_v, int *grid_lower, float dx, float inv_dx, float dt,
+             const int *grid_dim, float *out_particle_v, float ground_height, float *out_particle_C,
+             float *out_particle_x, int dim, float *particle_x_grad, float *grid_v_grad, float *out_particle_v_grad,
+             float *out_particle_C_grad, float *out_particle_x_grad, hipStream_t stream);
+
+/* replaces compute_dist, integrator.cu:1935-1960 */
+int compute_dist(float *particle_x, float *body_pos, float *body_rot, float *body_type_friction_softness_round,
+                 float *body_args, float *dist, int n_bodies, float *particle_x_grad, float *body_pos_grad,
+                 float *body_rot_grad, float *dist_grad, int compute_grad, int dim, hipStream_t stream);
+
+/* replaces particle2mass, integrator.cu:1962-1984 */
+int particle2mass(float *particle_x, float *particle_m, int *grid_lower, const int *grid_dim, float dx,
+                  float inv_dx, float *out_grid_m, float *out_grid_m_grad, float *particle_x_grad,
+                  int *particle_ids, int id, int compute_grad, int dim, hipStream_t stream);
+
+/* ---------------------------------------------------------------- group 2
+ * Multi-scene fused substep. All pointers in the structs are device pointers unless
+ * noted. Particles of all scenes are concatenated; grids are per scene, 4x4x4-blocked. */
+
+#define DDM_MAX_BODIES 64
+
+typedef struct ddm_scene {
+  int32_t p_begin, p_count;   /* particle range */
+  int32_t blk_begin, n_blocks;/* first 4^3 node block (grid arrays hold 64 nodes per block) */
+  int32_t dim[3];             /* grid nodes per axis, multiples of 4 */
+  int32_t n_bodies;
+  int32_t b_begin;            /* first body in the body arrays */
+  int32_t adjoint_mode;       /* 0 = reference adjoint, 1 = exact (finite-difference consistent) */
+  int32_t pad0, pad1;
+  float dx, inv_dx, dt, ground_friction;
+  float ground_height;        /* in cells (3 in the reference, mpm/simulator.py:493-494) */
+  float gravity[3];           /* as seen by the kernel: configured gravity * 30 (mpm/simulator.py:385) */
+} ddm_scene;                  /* 80 bytes */
+
+typedef struct ddm_batch {
+  int32_t n_scenes;
+  int32_t n_chunks;           /* particle chunks of <= 256 particles, none straddling scenes */
+  int32_t n_blocks;           /* total node blocks over all scenes */
+  int32_t max_bodies;         /* max n_bodies over scenes */
+  const ddm_scene *scenes;    /* [n_scenes] */
+  const int32_t *chunks;      /* [n_chunks][4] = (scene, p_begin, p_end, 0) */
+  const float *mass, *vol;    /* [P] */
+  const float *mu_lam_yield;  /* [P][3] */
+  const float *body_prim;     /* [NB][4] (type, friction, softness, round) */
+  const float *body_args;     /* [NB][4] */
+  float *grid_mmv;            /* [64*n_blocks][4] (m, mv) ; zero between substeps */
+  float *grid_v;              /* [64*n_blocks][4] node velocity after the grid update */
+  float *grid_dv;             /* [64*n_blocks][4] adjoint of grid_v ; zero between substeps */
+  float *grid_dmmv;           /* [64*n_blocks][4] adjoint of (m, mv) */
+  uint32_t *block_flags;      /* [n_blocks] ; zero between substeps */
+  float *tmp_Ft, *tmp_U, *tmp_V, *tmp_sig; /* [P][9],[P][9],[P][9],[P][3] ; backward only */
+} ddm_batch;
+
+typedef struct ddm_state {
+  float *x, *v, *F, *C;       /* [P][3], [P][3], [P][9], [P][9] */
+  float *body_pos, *body_rot; /* [NB][3], [NB][4] */
+} ddm_state;
+
+/* One forward substep of every scene: cur -> next (next->body_* must hold the poses of
+ * substep f+1). Writes next->x, v, F, C. Replaces MPMSimulator.substep. */
+int ddm_substep_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream);
+
+/* Adjoint of one substep of every scene. Reads cur, next (state and poses) and
+ * next_grad->x, v, F, C; ADDS into cur_grad->x, v, F, C, body_pos, body_rot and into
+ * next_grad->body_pos, body_rot. Replaces MPMSimulator.substep_grad. */
+int ddm_substep_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
+                         const ddm_state *next_grad, hipStream_t stream);
+
+/* Instrumented variants: identical work, plus hipEventRecord on `stream` around each
+ * stage kernel (events[0..3] for the forward: before P2G, after P2G, after the grid
+ * update, after G2P; events[0..5] for the backward: before the P2G recompute, after
+ * it, after the grid update, after the G2P adjoint, after the grid adjoint, after the
+ * P2G adjoint). Used by the benchmark to time each kernel on the stream it runs on. */
+int ddm_substep_forward_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream,
+                              hipEvent_t *events);
+int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                               const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
+                               hipEvent_t *events);
+
+/* Zeroes the batch's grid work arrays and block flags (after an error or on setup). */
+int ddm_batch_reset(const ddm_batch *b, hipStream_t stream);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ddm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
