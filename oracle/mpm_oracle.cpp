@@ -293,8 +293,22 @@ inline double accurate_sqrt(double x) { return x / std::sqrt(x); }  // svd.h:22
 inline void cond_swap(bool c, double &X, double &Y) { double Z = X; X = c ? Y : X; Y = c ? Z : Y; }          // svd.h:24-29
 inline void cond_neg_swap(bool c, double &X, double &Y) { double Z = -X; X = c ? Y : X; Y = c ? Z : Y; }   // svd.h:31-36
 
+// SVD mode: 0 = the reference's McAdams approximate Givens (default, parity);
+// 1 = exact Jacobi rotations with a renormalised quaternion. Mode 1 is a smooth
+// function of A (away from repeated singular values) and is used only by the
+// finite-difference checks of the adjoints, which differentiate the exact SVD.
+static int g_svd_mode = 0;
+
 // svd.h:111-124
 inline void approx_givens(double a11, double a12, double a22, double &ch, double &sh) {
+  if (g_svd_mode == 1) {
+    // inner rotation |phi| <= pi/4 (the classical convergent Jacobi choice)
+    double d = a11 - a22;
+    double phi = d != 0.0 ? 0.5 * std::atan(2 * a12 / d) : (a12 >= 0 ? 0.25 : -0.25) * 3.14159265358979323846;
+    ch = std::cos(0.5 * phi);
+    sh = std::sin(0.5 * phi);
+    return;
+  }
   ch = 2 * (a11 - a22);
   sh = a12;
   bool b = kGamma * sh * sh < ch * ch;
@@ -355,10 +369,15 @@ inline void svd3_d(const double A[3][3], double U[3][3], double sig[3], double V
     for (int j = 0; j < 3; ++j) ATA[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
   double s[6] = {ATA[0][0], ATA[1][0], ATA[1][1], ATA[2][0], ATA[2][1], ATA[2][2]};
   double q[4] = {0, 0, 0, 1};
-  for (int it = 0; it < 8; ++it) {  // svd.h:193-210
+  const int sweeps = g_svd_mode == 1 ? 12 : 8;
+  for (int it = 0; it < sweeps; ++it) {  // svd.h:193-210 (8 sweeps)
     jacobi_conj(0, 1, 2, s, q);
     jacobi_conj(1, 2, 0, s, q);
     jacobi_conj(2, 0, 1, s, q);
+  }
+  if (g_svd_mode == 1) {
+    double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i) q[i] /= qn;
   }
   quat_to_mat(q, V);
   double B[3][3];  // B = A V (svd.h:39-58)
@@ -1140,6 +1159,9 @@ void g2p_grad(int n, const T *x, const T *grid_v, const Grid &g, T dx, T inv_dx,
 
 ORC_INSTANTIATE(float, f32)
 ORC_INSTANTIATE(double, f64)
+
+ORC_EXPORT void orc_set_svd_mode(int mode) { orc::g_svd_mode = mode; }
+ORC_EXPORT int orc_get_svd_mode(void) { return orc::g_svd_mode; }
 
 ORC_EXPORT int orc_num_threads(void) {
 #ifdef _OPENMP

@@ -88,6 +88,21 @@ class _F:
         self.v = float(v)
 
 
+class exact_svd:
+    """Context manager: use the exact (smooth) Jacobi SVD in the oracle instead of the
+    reference's approximate one. For finite-difference checks only."""
+
+    def __enter__(self):
+        for lib in (_lib(False), _lib(True)):
+            lib.orc_set_svd_mode(1)
+        return self
+
+    def __exit__(self, *exc):
+        for lib in (_lib(False), _lib(True)):
+            lib.orc_set_svd_mode(0)
+        return False
+
+
 def num_threads() -> int:
     fn = _lib(True).orc_num_threads
     fn.restype = ctypes.c_int
