@@ -13,7 +13,8 @@ import os
 import numpy as np
 import torch  # noqa: F401  (must precede loading the HIP library)
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libdexdeform_mpm.so")
+LIB_PATH = os.environ.get("DDM_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
+                                                     "libdexdeform_mpm.so")
 
 c_int32 = ctypes.c_int32
 c_float = ctypes.c_float

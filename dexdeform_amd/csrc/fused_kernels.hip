@@ -146,21 +146,24 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, ddm_state cur, ddm_sta
     for (int i = threadIdx.x; i < T.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
     __syncthreads();
     if (valid) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            float cm;
-            f3 cmv;
-            p2g_contrib(q, S.dx, i, j, k, cm, cmv);
-            int li = T.local(q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
-            atomicAdd(&tm[li], cm);
-            atomicAdd(&tx[li], cmv.x);
-            atomicAdd(&ty[li], cmv.y);
-            atomicAdd(&tz[li], cmv.z);
-          }
+      // lanes start at different stencil offsets: particles sharing a cell (adjacent
+      // in the sorted order) then hit distinct LDS addresses in each instruction
+      const int start = (int)(threadIdx.x % 27u);
+#pragma unroll 1
+      for (int oo = 0; oo < 27; ++oo) {
+        int o = oo + start;
+        o = o >= 27 ? o - 27 : o;
+        int i, j, k;
+        offset_ijk(o, i, j, k);
+        float cm;
+        f3 cmv;
+        p2g_contrib(q, S.dx, i, j, k, cm, cmv);
+        int li = T.local(q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
+        atomicAdd(&tm[li], cm);
+        atomicAdd(&tx[li], cmv.x);
+        atomicAdd(&ty[li], cmv.y);
+        atomicAdd(&tz[li], cmv.z);
+      }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < T.n; i += kWG) {
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, ddm_state cur, ddm
       TileScatter3 sc{tx, ty, tz, &T};
       f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
                                 ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc, S.inv_dx,
-                                S.dx, S.dt, S.ground_height, dim);
+                                S.dx, S.dt, S.ground_height, dim, (int)(threadIdx.x % 27u));
       add3(cur_grad.x + 3 * (size_t)p, gxp);
     }
     __syncthreads();

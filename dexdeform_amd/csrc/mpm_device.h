@@ -540,9 +540,23 @@ DDM_HD void p2g_particle(f3 x, f3 v, const m33 &C, const m33 &Ftmp, const m33 &U
   out.m = m;
   (void)dx;
 }
+// select among three values by a small runtime index (keeps stencil tables in registers)
+DDM_HD float sel3(int i, float a, float b, float c) { return i == 0 ? a : (i == 1 ? b : c); }
+// o in [0, 27) -> stencil offset (i, j, k), i-major as the reference loops
+DDM_HD void offset_ijk(int o, int &i, int &j, int &k) {
+  i = o / 9;
+  j = (o / 3) % 3;
+  k = o % 3;
+}
+// stencil weight for a runtime offset
+DDM_HD float weight_dyn(const Stencil &s, int i, int j, int k) {
+  return sel3(i, s.w[0][0], s.w[0][1], s.w[0][2]) * sel3(j, s.w[1][0], s.w[1][1], s.w[1][2]) *
+         sel3(k, s.w[2][0], s.w[2][1], s.w[2][2]);
+}
+
 // contribution of one particle to stencil node (i,j,k): (m w, w (m v + affine dpos))
 DDM_HD void p2g_contrib(const P2GParticle &p, float dx, int i, int j, int k, float &cm, f3 &cmv) {
-  float w = p.st.w[0][i] * p.st.w[1][j] * p.st.w[2][k];
+  float w = weight_dyn(p.st, i, j, k);
   f3 dpos = f3{((float)i - p.st.fx[0]) * dx, ((float)j - p.st.fx[1]) * dx, ((float)k - p.st.fx[2]) * dx};
   cm = p.m * w;
   cmv = (p.mv + mvec(p.affine, dpos)) * w;

@@ -10,6 +10,7 @@ Restates the reference's host-side kinematics without pytorch3d:
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 
@@ -120,3 +121,93 @@ def rigid_body_motion_hand(state: torch.Tensor, actions: torch.Tensor, T: int) -
     out = out / torch.linalg.norm(out, dim=-1, keepdim=True)
     state[..., :3, :3] = quaternion_to_matrix(out)
     return state
+
+
+# ---------------------------------------------------------------- articulated capsule hands
+class HingeTree:
+    """Kinematic tree of hinge joints carrying one collision primitive per link.
+
+    Joint-velocity control as in HandSimulator.JointVel_Fk (mpm/hand.py:383-428):
+    q_i = clamp(q0 + (clamp(a, -1, 1) * scale)[action_map] * (i + 1), lower, upper) for
+    substep i (no division by the substep count, mpm/hand.py:414-415), then batched
+    forward kinematics T_link = T_parent * Trans(offset) * Rot(axis, q) and
+    geom = T_link * geom_offset (mpm/hand.py:347-381). Link order must be topological
+    (parent index < link index, -1 = the base frame).
+    """
+
+    def __init__(self, parents, offsets, axes, geom_offsets, lower, upper, base, scale, action_map=None,
+                 device="cpu"):
+        f = lambda a: torch.as_tensor(a, dtype=torch.float32, device=device)
+        self.parents = list(int(p) for p in parents)
+        self.n = len(self.parents)
+        assert all(p < i for i, p in enumerate(self.parents))
+        self.offsets = f(offsets).reshape(self.n, 3)
+        self.axes = f(axes).reshape(self.n, 3)
+        self.axes = self.axes / torch.linalg.norm(self.axes, dim=-1, keepdim=True)
+        self.geom = f(geom_offsets).reshape(self.n, 4, 4)
+        self.lower = f(lower).reshape(self.n)
+        self.upper = f(upper).reshape(self.n)
+        self.base = f(base).reshape(4, 4)
+        self.scale = f(scale).reshape(-1)
+        self.action_map = (torch.as_tensor(action_map, dtype=torch.long, device=device) if action_map is not None
+                           else torch.arange(self.n, device=device))
+
+    def fk(self, q: torch.Tensor):
+        """q [..., n] joint angles -> (pos [..., n, 3], quat [..., n, 4]) of the primitives."""
+        R = axis_angle_to_matrix(self.axes * q[..., None])
+        batch = q.shape[:-1]
+        T = torch.zeros(batch + (self.n, 4, 4), dtype=q.dtype, device=q.device)
+        T[..., :3, :3] = R
+        T[..., :3, 3] = self.offsets
+        T[..., 3, 3] = 1.0
+        frames = []
+        for i, p in enumerate(self.parents):
+            parent = self.base.expand(batch + (4, 4)) if p < 0 else frames[p]
+            frames.append(parent @ T[..., i, :, :])
+        G = torch.stack(frames, -3) @ self.geom
+        return G[..., :3, 3], matrix_to_quaternion(G[..., :3, :3])
+
+    def joint_targets(self, q0: torch.Tensor, action: torch.Tensor, substeps: int):
+        a = (action.reshape(-1).clamp(-1.0, 1.0) * self.scale)[self.action_map]
+        ramp = torch.arange(substeps, device=action.device, dtype=action.dtype)[:, None] + 1
+        return torch.minimum(torch.maximum(q0[None] + a[None] * ramp, self.lower), self.upper)
+
+    def poses(self, q0: torch.Tensor, action: torch.Tensor, substeps: int):
+        """Per-substep primitive poses for one joint-velocity action; returns
+        (pos [S, n, 3], rot [S, n, 4], q at the last substep)."""
+        q = self.joint_targets(q0, action, substeps)
+        pos, rot = self.fk(q)
+        return pos, rot, q[-1]
+
+
+def synthetic_hand(seed: int = 4, palm=(0.5, 0.34, 0.5), device="cpu"):
+    """A hand-like hinge tree of 22 capsule links: 2 palm links hanging from a fixed base
+    and 5 fingers of 4 links each, pointing down (-y). Capsule radius U[0.008, 0.015],
+    half-length U[0.01, 0.025] (segment length = 2 * half-length); each capsule starts at
+    its joint and the next link's joint sits at the end of its segment.
+    Returns (tree, capsule args [22, 4], initial joint angles [22])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 22
+    r = rng.uniform(0.008, 0.015, n)
+    hl = rng.uniform(0.01, 0.025, n)
+    parents = [-1, -1]
+    offsets = [[0.0, 0.0, -0.02], [0.0, 0.0, 0.02]]
+    axes = [[1.0, 0.0, 0.0], [1.0, 0.0, 0.0]]
+    for fi in range(5):
+        prev = fi % 2
+        for li in range(4):
+            parents.append(prev)
+            offsets.append([(fi - 2) * 0.025, -0.01, 0.0] if li == 0 else [0.0, -2 * hl[prev], 0.0])
+            axes.append([0.0, 0.0, 1.0] if li == 0 else [1.0, 0.0, 0.0])
+            prev = len(parents) - 1
+    geoms = np.tile(np.eye(4), (n, 1, 1))
+    geoms[:, 1, 3] = -hl
+    base = np.eye(4)
+    base[:3, 3] = palm
+    tree = HingeTree(parents, np.asarray(offsets), np.asarray(axes), geoms, np.full(n, -0.6), np.full(n, 0.6), base,
+                     np.full(n, 0.33 * 0.002), device=device)
+    args = np.zeros((n, 4), np.float32)
+    args[:, 0] = r
+    args[:, 1] = 2 * hl
+    q0 = torch.as_tensor(rng.uniform(-0.2, 0.2, size=n), dtype=torch.float32, device=device)
+    return tree, args, q0
