@@ -111,9 +111,8 @@ class KernelTimer:
         assert ro.full, "kernel timing uses the non-checkpointed rollout"
         for t in range(ro.T):
             evs, arr = events(4)
-            _lib.check(lib.ddm_substep_forward_timed(ctypes.byref(eng._batch),
-                                                     ctypes.byref(ro._sview(t, t).ctype()),
-                                                     ctypes.byref(ro._sview(t + 1, t + 1).ctype()), st, arr))
+            _lib.check(lib.ddm_substep_forward_timed(ctypes.byref(eng._batch), ro._sview(t, t).data_ptr(),
+                                                     ro._sview(t + 1, t + 1).data_ptr(), st, arr))
             fwd_ev.append((evs, arr))
         gn = ro.g[0]
         gn.x.copy_(gx)
@@ -128,9 +127,8 @@ class KernelTimer:
             ro.g[cur].zero_()
             evs, arr = events(6)
             _lib.check(lib.ddm_substep_backward_timed(
-                ctypes.byref(eng._batch), ctypes.byref(ro._sview(t, t).ctype()),
-                ctypes.byref(ro._sview(t + 1, t + 1).ctype()), ctypes.byref(ro._gview(cur, t).ctype()),
-                ctypes.byref(ro._gview(nxt, t + 1).ctype()), st, arr))
+                ctypes.byref(eng._batch), ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(),
+                ro._gview(cur, t).data_ptr(), ro._gview(nxt, t + 1).data_ptr(), st, arr))
             bwd_ev.append((evs, arr))
         torch.cuda.synchronize()
         tot = {k: 0.0 for k in FWD_NAMES + BWD_NAMES}
@@ -185,6 +183,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)",
@@ -193,6 +193,13 @@ def main():
 
     import torch
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+        del src, dst
     T = args.substeps
     scenes, poses = make_batched(args.scenes, args.seed, T)
     n_total = sum(s.n for s in scenes)

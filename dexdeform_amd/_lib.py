@@ -29,38 +29,42 @@ ERROR_NAMES = {DDM_EINVAL: "DDM_EINVAL (invalid argument)", DDM_ETOOMANYBODIES: 
 
 class DdmScene(ctypes.Structure):
     _fields_ = [
-        ("p_begin", c_int32), ("p_count", c_int32), ("blk_begin", c_int32), ("n_blocks", c_int32),
-        ("dim", c_int32 * 3), ("n_bodies", c_int32), ("b_begin", c_int32), ("adjoint_mode", c_int32),
-        ("pad0", c_int32), ("pad1", c_int32),
+        ("p_count", c_int32), ("blk_begin", c_int32), ("n_blocks", c_int32), ("dim", c_int32 * 3),
+        ("n_bodies", c_int32), ("adjoint_mode", c_int32),
         ("dx", c_float), ("inv_dx", c_float), ("dt", c_float), ("ground_friction", c_float),
         ("ground_height", c_float), ("gravity", c_float * 3),
+        ("mass", c_void_p), ("vol", c_void_p), ("mu_lam_yield", c_void_p), ("body_prim", c_void_p),
+        ("body_args", c_void_p), ("tmp_Ft", c_void_p), ("tmp_U", c_void_p), ("tmp_V", c_void_p), ("tmp_sig", c_void_p),
     ]
 
 
 SCENE_DTYPE = np.dtype([
-    ("p_begin", "<i4"), ("p_count", "<i4"), ("blk_begin", "<i4"), ("n_blocks", "<i4"),
-    ("dim", "<i4", (3,)), ("n_bodies", "<i4"), ("b_begin", "<i4"), ("adjoint_mode", "<i4"),
-    ("pad0", "<i4"), ("pad1", "<i4"),
+    ("p_count", "<i4"), ("blk_begin", "<i4"), ("n_blocks", "<i4"), ("dim", "<i4", (3,)),
+    ("n_bodies", "<i4"), ("adjoint_mode", "<i4"),
     ("dx", "<f4"), ("inv_dx", "<f4"), ("dt", "<f4"), ("ground_friction", "<f4"),
     ("ground_height", "<f4"), ("gravity", "<f4", (3,)),
+    ("mass", "<u8"), ("vol", "<u8"), ("mu_lam_yield", "<u8"), ("body_prim", "<u8"), ("body_args", "<u8"),
+    ("tmp_Ft", "<u8"), ("tmp_U", "<u8"), ("tmp_V", "<u8"), ("tmp_sig", "<u8"),
 ])
-assert SCENE_DTYPE.itemsize == ctypes.sizeof(DdmScene) == 80
+assert SCENE_DTYPE.itemsize == ctypes.sizeof(DdmScene) == 136
 
 
 class DdmBatch(ctypes.Structure):
     _fields_ = [
         ("n_scenes", c_int32), ("n_chunks", c_int32), ("n_blocks", c_int32), ("max_bodies", c_int32),
-        ("scenes", c_void_p), ("chunks", c_void_p), ("mass", c_void_p), ("vol", c_void_p),
-        ("mu_lam_yield", c_void_p), ("body_prim", c_void_p), ("body_args", c_void_p),
-        ("grid_mmv", c_void_p), ("grid_v", c_void_p), ("grid_dv", c_void_p), ("grid_dmmv", c_void_p),
-        ("block_flags", c_void_p), ("tmp_Ft", c_void_p), ("tmp_U", c_void_p), ("tmp_V", c_void_p),
-        ("tmp_sig", c_void_p),
+        ("scenes", c_void_p), ("chunks", c_void_p), ("grid_mmv", c_void_p), ("grid_v", c_void_p),
+        ("grid_dv", c_void_p), ("grid_dmmv", c_void_p), ("block_flags", c_void_p),
     ]
 
 
 class DdmState(ctypes.Structure):
     _fields_ = [("x", c_void_p), ("v", c_void_p), ("F", c_void_p), ("C", c_void_p),
                 ("body_pos", c_void_p), ("body_rot", c_void_p)]
+
+
+STATE_DTYPE = np.dtype([("x", "<u8"), ("v", "<u8"), ("F", "<u8"), ("C", "<u8"), ("body_pos", "<u8"),
+                        ("body_rot", "<u8")])
+assert STATE_DTYPE.itemsize == ctypes.sizeof(DdmState) == 48
 
 
 # Symbols declared in include/dexdeform_mpm.h
@@ -84,11 +88,11 @@ _ARGTYPES = {
     "g2p_grad": [_P, _P, _P, _F, _F, _F, _P, _P, _F, _P, _P, _I, _P, _P, _P, _P, _P, _P],
     "compute_dist": [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _I, _P],
     "particle2mass": [_P, _P, _P, _P, _F, _F, _P, _P, _P, _P, _I, _I, _I, _P],
-    "ddm_substep_forward": [ctypes.POINTER(DdmBatch), ctypes.POINTER(DdmState), ctypes.POINTER(DdmState), _P],
-    "ddm_substep_backward": [ctypes.POINTER(DdmBatch)] + [ctypes.POINTER(DdmState)] * 4 + [_P],
-    "ddm_substep_forward_timed": [ctypes.POINTER(DdmBatch), ctypes.POINTER(DdmState), ctypes.POINTER(DdmState),
-                                  _P, _P],
-    "ddm_substep_backward_timed": [ctypes.POINTER(DdmBatch)] + [ctypes.POINTER(DdmState)] * 4 + [_P, _P],
+    # state tables are device arrays of DdmState records: passed as raw device pointers
+    "ddm_substep_forward": [ctypes.POINTER(DdmBatch), _P, _P, _P],
+    "ddm_substep_backward": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P],
+    "ddm_substep_forward_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P],
+    "ddm_substep_backward_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
     "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],
     "ddm_version": [],
 }

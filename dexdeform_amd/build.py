@@ -26,6 +26,9 @@ HIPCC_FLAGS = [
     "-fPIC",
     # hardware FP32 atomic add on global memory (no CAS loop)
     "-munsafe-fp-atomics",
+    # no FMA contraction: the arithmetic then matches the CPU restatement operation for
+    # operation, leaving FP32 accumulation order as the only source of difference
+    "-ffp-contract=off",
     # keep the compiler from forming packed FP32 (v_pk_*_f32) instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",

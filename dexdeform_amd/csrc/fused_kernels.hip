@@ -1,8 +1,9 @@
 // Fused multi-scene substep for gfx950.
 //
 // Layout and invariants
-//  * Particles of all scenes are concatenated; a chunk table maps each workgroup to
-//    <= 256 particles of one scene.
+//  * Scenes are independent: per-scene records (ddm_scene, ddm_state) in device memory
+//    give each scene's particle arrays, material, primitives and poses; a chunk table
+//    maps each workgroup to <= 256 particles of one scene (local indices).
 //  * Each scene's grid is stored in 4x4x4 node blocks (64 nodes = one wavefront),
 //    blocks of all scenes concatenated; every grid array is float4 per node.
 //  * Block-sparse: P2G marks the blocks it touches in block_flags; the grid kernels
@@ -111,11 +112,13 @@ __device__ void flag_stencil_blocks(const ddm_scene &S, const int base[3], uint3
 // BWD=false: forward substep, writes next.F.
 // BWD=true : adjoint recompute, writes the F_tmp/U/V/sigma temporaries instead.
 template <bool BWD>
-__global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, ddm_state cur, ddm_state next) {
+__global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                             const ddm_state *__restrict__ next_tab) {
   __shared__ float tm[kTileMax], tx[kTileMax], ty[kTileMax], tz[kTileMax];
   __shared__ int s_box[6];
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   const bool valid = p < ch.z;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
@@ -127,15 +130,15 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, ddm_state cur, ddm_sta
     f3 sig;
     svd3(Ft, U, sig, V);
     m33 Fn;
-    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, B.mass[p], B.vol[p],
-                 ld3(B.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
+    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+                 ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
     if (BWD) {
-      st9(B.tmp_Ft + 9 * (size_t)p, Ft);
-      st9(B.tmp_U + 9 * (size_t)p, U);
-      st9(B.tmp_V + 9 * (size_t)p, V);
-      st3(B.tmp_sig + 3 * (size_t)p, sig);
+      st9(S.tmp_Ft + 9 * (size_t)p, Ft);
+      st9(S.tmp_U + 9 * (size_t)p, U);
+      st9(S.tmp_V + 9 * (size_t)p, V);
+      st3(S.tmp_sig + 3 * (size_t)p, sig);
     } else {
-      st9(next.F + 9 * (size_t)p, Fn);
+      st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     }
   } else {
     q.st.base[0] = q.st.base[1] = q.st.base[2] = 0;
@@ -217,7 +220,7 @@ __device__ __forceinline__ BlockCoord block_node(const ddm_scene &S, int blk, in
   int bx = lb / (nby * nbz), by = (lb / nbz) % nby, bz = lb % nbz;
   return BlockCoord{bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
 }
-__device__ __forceinline__ NodeParams scene_node_params(const ddm_batch &B, const ddm_scene &S, const ddm_state &cur,
+__device__ __forceinline__ NodeParams scene_node_params(const ddm_scene &S, const ddm_state &cur,
                                                         const ddm_state &next) {
   NodeParams P;
   P.dim[0] = S.dim[0];
@@ -228,31 +231,33 @@ __device__ __forceinline__ NodeParams scene_node_params(const ddm_batch &B, cons
   P.dt = S.dt;
   P.gf = S.ground_friction;
   P.gravity = f3{S.gravity[0], S.gravity[1], S.gravity[2]};
-  P.prim = B.body_prim + 4 * S.b_begin;
-  P.args = B.body_args + 4 * S.b_begin;
-  P.pos = cur.body_pos + 3 * S.b_begin;
-  P.rot = cur.body_rot + 4 * S.b_begin;
-  P.npos = next.body_pos + 3 * S.b_begin;
-  P.nrot = next.body_rot + 4 * S.b_begin;
+  P.prim = S.body_prim;
+  P.args = S.body_args;
+  P.pos = cur.body_pos;
+  P.rot = cur.body_rot;
+  P.npos = next.body_pos;
+  P.nrot = next.body_rot;
   return P;
 }
 
 // ---------------------------------------------------------------- grid update
 // ZERO=true (forward substep): consumes (m, mv) and clears them and the flags.
 template <bool ZERO>
-__global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, ddm_state cur, ddm_state next) {
+__global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                              const ddm_state *__restrict__ next_tab) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
   for (int blk = blockIdx.x * (kWG / 64) + wave; blk < B.n_blocks; blk += gridDim.x * (kWG / 64)) {
     if (B.block_flags[blk] == 0u) continue;
-    const ddm_scene &S = B.scenes[scene_of_block(B.scenes, B.n_scenes, blk)];
+    const int si = scene_of_block(B.scenes, B.n_scenes, blk);
+    const ddm_scene &S = B.scenes[si];
     BlockCoord c = block_node(S, blk, lane);
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
     float4 a = mmv[gi];
     f3 v{0.f, 0.f, 0.f};
     if (node_active(a.x)) {
-      NodeParams P = scene_node_params(B, S, cur, next);
+      NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
       v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P);
     }
     gv[gi] = make_float4(v.x, v.y, v.z, 0.f);
@@ -270,15 +275,18 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 // adjoint of the grid update; clears (m, mv), the velocity adjoint and the flags
-__global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_state cur, ddm_state next, ddm_state cur_grad,
-                                                  ddm_state next_grad) {
+__global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                  const ddm_state *__restrict__ next_tab,
+                                                  const ddm_state *__restrict__ cur_grad_tab,
+                                                  const ddm_state *__restrict__ next_grad_tab) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
   float4 *dmmv = reinterpret_cast<float4 *>(B.grid_dmmv);
   for (int blk = blockIdx.x * (kWG / 64) + wave; blk < B.n_blocks; blk += gridDim.x * (kWG / 64)) {
     if (B.block_flags[blk] == 0u) continue;
-    const ddm_scene &S = B.scenes[scene_of_block(B.scenes, B.n_scenes, blk)];
+    const int si = scene_of_block(B.scenes, B.n_scenes, blk);
+    const ddm_scene &S = B.scenes[si];
     BlockCoord c = block_node(S, blk, lane);
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
     float4 a = mmv[gi];
@@ -286,7 +294,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_state cur, dd
     const float m = a.x;
     const f3 mv{a.y, a.z, a.w};
     const bool act = node_active(m);
-    NodeParams P = scene_node_params(B, S, cur, next);
+    NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
     const f3 gx = node_position(c.ix, c.iy, c.iz, P.dx);
     const int mode = S.adjoint_mode;
     // forward replay keeping the velocities entering each contact (integrator.cu:678-732)
@@ -334,9 +342,10 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_state cur, dd
 #pragma unroll
         for (int e = 0; e < 14; ++e) vals[e] = wave_sum(vals[e]);
         if (lane == 0) {
-          int gb = S.b_begin + b;
-          float *cp = cur_grad.body_pos + 3 * gb, *cr = cur_grad.body_rot + 4 * gb;
-          float *np = next_grad.body_pos + 3 * gb, *nr = next_grad.body_rot + 4 * gb;
+          const ddm_state &cg = cur_grad_tab[si];
+          const ddm_state &ng = next_grad_tab[si];
+          float *cp = cg.body_pos + 3 * b, *cr = cg.body_rot + 4 * b;
+          float *np = ng.body_pos + 3 * b, *nr = ng.body_rot + 4 * b;
           atomicAdd(cp + 0, vals[0]);
           atomicAdd(cp + 1, vals[1]);
           atomicAdd(cp + 2, vals[2]);
@@ -367,9 +376,11 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_state cur, dd
 }
 
 // ---------------------------------------------------------------- G2P
-__global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, ddm_state cur, ddm_state next) {
+__global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                             const ddm_state *__restrict__ next_tab) {
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   if (p >= ch.z) return;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
@@ -404,12 +415,16 @@ struct GlobalScatter3 {
 };
 
 // adjoint of G2P: scatters the node-velocity adjoint, adds to the x adjoint
-__global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, ddm_state cur, ddm_state next, ddm_state cur_grad,
-                                                 ddm_state next_grad) {
+__global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                 const ddm_state *__restrict__ next_tab,
+                                                 const ddm_state *__restrict__ cur_grad_tab,
+                                                 const ddm_state *__restrict__ next_grad_tab) {
   __shared__ float tx[kTileMax], ty[kTileMax], tz[kTileMax];
   __shared__ int s_box[6];
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
+  const ddm_state cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   const bool valid = p < ch.z;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
@@ -453,21 +468,23 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, ddm_state cur, ddm
 }
 
 // adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
-__global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, ddm_state cur, ddm_state cur_grad,
-                                                 ddm_state next_grad) {
+__global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                 const ddm_state *__restrict__ cur_grad_tab,
+                                                 const ddm_state *__restrict__ next_grad_tab) {
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   if (p >= ch.z) return;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   const size_t p3 = 3 * (size_t)p, p9 = 9 * (size_t)p;
   BlockedGatherDMMV gat{reinterpret_cast<const float4 *>(B.grid_dmmv), &S};
   m33 F = ld9(cur.F + p9), C = ld9(cur.C + p9);
-  m33 Ft = ld9(B.tmp_Ft + p9), U = ld9(B.tmp_U + p9), V = ld9(B.tmp_V + p9);
-  f3 sig = ld3(B.tmp_sig + p3);
+  m33 Ft = ld9(S.tmp_Ft + p9), U = ld9(S.tmp_U + p9), V = ld9(S.tmp_V + p9);
+  f3 sig = ld3(S.tmp_sig + p3);
   P2GAdj a;
-  p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, B.mass[p], B.vol[p],
-                   ld3(B.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
+  p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+                   ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
   m33 gFt, gF, gC;
   svd_adj(F, C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
   add3(cur_grad.x + p3, a.x);
@@ -490,8 +507,8 @@ inline int grid_wgs(const ddm_batch *b) {
   int w = (b->n_blocks + (kWG / 64) - 1) / (kWG / 64);
   return w < 4096 ? (w > 0 ? w : 1) : 4096;
 }
-inline int check(const ddm_batch *b, const ddm_state *s) {
-  if (!b || !s || b->n_chunks < 0 || b->n_blocks < 0 || b->n_scenes <= 0) return DDM_EINVAL;
+inline int check(const ddm_batch *b, const void *cur, const void *next) {
+  if (!b || !cur || !next || b->n_chunks < 0 || b->n_blocks < 0 || b->n_scenes <= 0) return DDM_EINVAL;
   if (b->max_bodies > DDM_MAX_BODIES) return DDM_ETOOMANYBODIES;
   if (!b->scenes || !b->chunks || !b->grid_mmv || !b->grid_v || !b->block_flags) return DDM_EINVAL;
   return 0;
@@ -507,38 +524,35 @@ static inline void mark(hipEvent_t *ev, int i, hipStream_t stream) {
 
 static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream,
                           hipEvent_t *ev) {
-  int e = check(b, cur);
+  int e = check(b, cur, next);
   if (e) return e;
-  if (!next) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<false><<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next);
+  k_p2g<false><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<true><<<grid_wgs(b), kWG, 0, stream>>>(*b, *cur, *next);
+  k_grid<true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 2, stream);
-  k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next);
+  k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
   return (int)hipGetLastError();
 }
 
 static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                            const ddm_state *next_grad, hipStream_t stream, hipEvent_t *ev) {
-  int e = check(b, cur);
+  int e = check(b, cur, next);
   if (e) return e;
-  if (!next || !cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv || !b->tmp_Ft || !b->tmp_U || !b->tmp_V ||
-      !b->tmp_sig)
-    return DDM_EINVAL;
+  if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<true><<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next);
+  k_p2g<true><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<false><<<grid_wgs(b), kWG, 0, stream>>>(*b, *cur, *next);
+  k_grid<false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 2, stream);
-  k_g2p_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *next, *cur_grad, *next_grad);
+  k_g2p_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, cur_grad, next_grad);
   mark(ev, 3, stream);
-  k_grid_adj<<<grid_wgs(b), kWG, 0, stream>>>(*b, *cur, *next, *cur_grad, *next_grad);
+  k_grid_adj<<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, cur_grad, next_grad);
   mark(ev, 4, stream);
-  k_p2g_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, *cur, *cur_grad, *next_grad);
+  k_p2g_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, cur_grad, next_grad);
   mark(ev, 5, stream);
   return (int)hipGetLastError();
 }

@@ -72,7 +72,8 @@ def lame(E: float, nu: float):
 
 
 class StateBuffers:
-    """Particle state (x, v, F, C) of every scene of an engine plus body poses."""
+    """Particle state (x, v, F, C) of every scene of an engine plus body poses, stored
+    concatenated (scene s owns rows p_begin[s] : p_begin[s] + capacity[s])."""
 
     def __init__(self, n_particles: int, n_bodies: int, device, identity_F: bool = True):
         z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=device)
@@ -86,15 +87,9 @@ class StateBuffers:
         self.body_rot = z(max(n_bodies, 0), 4)
         if n_bodies:
             self.body_rot[:, 0] = 1.0
-        self._ct = None
 
     def tensors(self):
         return (self.x, self.v, self.F, self.C, self.body_pos, self.body_rot)
-
-    def ctype(self) -> _lib.DdmState:
-        if self._ct is None:
-            self._ct = _lib.DdmState(*(_lib.ptr(t) for t in self.tensors()))
-        return self._ct
 
     def zero_(self):
         for t in self.tensors():
@@ -107,13 +102,23 @@ class StateBuffers:
         return self
 
 
-def chunk_table(p_begins: Sequence[int], counts: Sequence[int]) -> np.ndarray:
-    """(scene, begin, end, 0) rows covering each scene's particles in <= CHUNK pieces."""
+def chunk_table(counts: Sequence[int], scene_ids: Optional[Sequence[int]] = None) -> np.ndarray:
+    """(scene, first, end, 0) rows covering each scene's particles (local indices) in
+    <= CHUNK pieces."""
     rows = []
-    for s, (b, n) in enumerate(zip(p_begins, counts)):
+    ids = range(len(counts)) if scene_ids is None else scene_ids
+    for s, n in zip(ids, counts):
         for c in range(0, int(n), CHUNK):
-            rows.append((s, b + c, b + min(c + CHUNK, int(n)), 0))
+            rows.append((s, c, min(c + CHUNK, int(n)), 0))
     return np.asarray(rows, dtype=np.int32).reshape(-1, 4)
+
+
+def state_table(records, device) -> torch.Tensor:
+    """Device array of ddm_state records from (x, v, F, C, body_pos, body_rot) address tuples."""
+    tab = np.zeros(len(records), dtype=_lib.STATE_DTYPE)
+    for i, r in enumerate(records):
+        tab[i] = tuple(int(a) for a in r)
+    return torch.as_tensor(tab.view(np.uint8).copy(), device=device)
 
 
 class Engine:
@@ -167,14 +172,14 @@ class Engine:
             self.grid_dv = self.grid_dmmv = self.tmp_Ft = self.tmp_U = self.tmp_V = self.tmp_sig = None
 
         tab = np.zeros(len(scenes), dtype=_lib.SCENE_DTYPE)
+        f4 = 4
         for i, s in enumerate(scenes):
-            tab[i]["p_begin"] = self.p_begin[i]
+            pb, bb = int(self.p_begin[i]), int(self.b_begin[i])
             tab[i]["p_count"] = s.n_particles
             tab[i]["blk_begin"] = self.blk_begin[i]
             tab[i]["n_blocks"] = nblk[i]
             tab[i]["dim"] = [int(g) for g in s.grid_dim]
             tab[i]["n_bodies"] = s.n_bodies
-            tab[i]["b_begin"] = self.b_begin[i]
             tab[i]["adjoint_mode"] = int(s.adjoint_mode)
             tab[i]["dx"] = s.dx
             tab[i]["inv_dx"] = 1.0 / float(s.dx)
@@ -182,17 +187,29 @@ class Engine:
             tab[i]["ground_friction"] = s.ground_friction
             tab[i]["ground_height"] = s.ground_height
             tab[i]["gravity"] = np.asarray(s.gravity, np.float32)
+            tab[i]["mass"] = self.mass.data_ptr() + pb * f4
+            tab[i]["vol"] = self.vol.data_ptr() + pb * f4
+            tab[i]["mu_lam_yield"] = self.mu_lam_yield.data_ptr() + 3 * pb * f4
+            if NB:
+                tab[i]["body_prim"] = self.body_prim.data_ptr() + 4 * bb * f4
+                tab[i]["body_args"] = self.body_args.data_ptr() + 4 * bb * f4
+            if with_grad:
+                tab[i]["tmp_Ft"] = self.tmp_Ft.data_ptr() + 9 * pb * f4
+                tab[i]["tmp_U"] = self.tmp_U.data_ptr() + 9 * pb * f4
+                tab[i]["tmp_V"] = self.tmp_V.data_ptr() + 9 * pb * f4
+                tab[i]["tmp_sig"] = self.tmp_sig.data_ptr() + 3 * pb * f4
         self._scene_np = tab
         self.scene_table = torch.as_tensor(tab.view(np.uint8).copy(), device=dev)
         self.counts = list(caps)
         self.chunks = None
+        self._tables = {}
         self._set_chunks(self.counts)
         self._batch = None
         self._rebuild_batch()
 
     # ------------------------------------------------------------------ tables
     def _set_chunks(self, counts):
-        tab = chunk_table(self.p_begin, counts)
+        tab = chunk_table(counts)
         self.n_chunks = int(len(tab))
         self.chunks = torch.as_tensor(tab if len(tab) else np.zeros((1, 4), np.int32), device=self.device)
 
@@ -211,10 +228,25 @@ class Engine:
         p = _lib.ptr
         self._batch = _lib.DdmBatch(
             len(self.scenes), self.n_chunks, self.n_blocks, max(s.n_bodies for s in self.scenes),
-            p(self.scene_table), p(self.chunks), p(self.mass), p(self.vol), p(self.mu_lam_yield),
-            p(self.body_prim) if self.n_bodies_total else None, p(self.body_args) if self.n_bodies_total else None,
-            p(self.grid_mmv), p(self.grid_v), p(self.grid_dv), p(self.grid_dmmv), p(self.block_flags),
-            p(self.tmp_Ft), p(self.tmp_U), p(self.tmp_V), p(self.tmp_sig))
+            p(self.scene_table), p(self.chunks), p(self.grid_mmv), p(self.grid_v), p(self.grid_dv), p(self.grid_dmmv),
+            p(self.block_flags))
+
+    def table(self, x, v, F, C, body_pos, body_rot) -> torch.Tensor:
+        """Per-scene ddm_state records for concatenated tensors (cached by address)."""
+        key = tuple(int(t.data_ptr()) for t in (x, v, F, C, body_pos, body_rot))
+        tab = self._tables.get(key)
+        if tab is None:
+            recs = []
+            for i in range(len(self.scenes)):
+                pb, bb = int(self.p_begin[i]), int(self.b_begin[i])
+                recs.append((key[0] + 12 * pb, key[1] + 12 * pb, key[2] + 36 * pb, key[3] + 36 * pb,
+                             key[4] + 12 * bb if key[4] else 0, key[5] + 16 * bb if key[5] else 0))
+            tab = state_table(recs, self.device)
+            self._tables[key] = tab
+        return tab
+
+    def table_of(self, buf) -> torch.Tensor:
+        return self.table(*buf.tensors())
 
     def update_body_params(self, body_prim: torch.Tensor = None, body_args: torch.Tensor = None):
         if body_prim is not None:
@@ -238,18 +270,24 @@ class Engine:
         return slice(b, b + self.scenes[i].n_bodies)
 
     # ------------------------------------------------------------------ substeps
-    def forward(self, cur: StateBuffers, nxt: StateBuffers, stream=None):
-        st = _lib.stream_ptr(stream)
-        _lib.check(self.lib.ddm_substep_forward(ctypes.byref(self._batch), ctypes.byref(cur.ctype()),
-                                                ctypes.byref(nxt.ctype()), st), "ddm_substep_forward")
+    def forward(self, cur, nxt, stream=None):
+        """One forward substep of every scene; cur/nxt are StateBuffers (or anything
+        with x, v, F, C, body_pos, body_rot tensors laid out like them)."""
+        self.forward_tables(self.table_of(cur), self.table_of(nxt), stream)
 
-    def backward(self, cur: StateBuffers, nxt: StateBuffers, gcur: StateBuffers, gnxt: StateBuffers, stream=None):
+    def backward(self, cur, nxt, gcur, gnxt, stream=None):
+        self.backward_tables(self.table_of(cur), self.table_of(nxt), self.table_of(gcur), self.table_of(gnxt), stream)
+
+    def forward_tables(self, cur_tab, nxt_tab, stream=None):
+        _lib.check(self.lib.ddm_substep_forward(ctypes.byref(self._batch), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+                                                _lib.stream_ptr(stream)), "ddm_substep_forward")
+
+    def backward_tables(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, stream=None):
         if not self.with_grad:
             raise RuntimeError("engine was created with with_grad=False")
-        st = _lib.stream_ptr(stream)
-        _lib.check(self.lib.ddm_substep_backward(ctypes.byref(self._batch), ctypes.byref(cur.ctype()),
-                                                 ctypes.byref(nxt.ctype()), ctypes.byref(gcur.ctype()),
-                                                 ctypes.byref(gnxt.ctype()), st), "ddm_substep_backward")
+        _lib.check(self.lib.ddm_substep_backward(ctypes.byref(self._batch), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+                                                 gcur_tab.data_ptr(), gnxt_tab.data_ptr(), _lib.stream_ptr(stream)),
+                   "ddm_substep_backward")
 
     def reset_grid(self, stream=None):
         _lib.check(self.lib.ddm_batch_reset(ctypes.byref(self._batch), _lib.stream_ptr(stream)), "ddm_batch_reset")

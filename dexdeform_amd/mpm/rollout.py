@@ -17,26 +17,12 @@ device copy and nothing crosses the host per substep. Pose gradients accumulate 
 """
 from __future__ import annotations
 
-import ctypes
 from typing import List, Optional
 
 import torch
 
 from .. import _lib
 from .engine import Engine
-
-
-class _View:
-    """A DdmState whose particle arrays and pose rows come from different tensors."""
-
-    __slots__ = ("x", "v", "F", "C", "body_pos", "body_rot", "_ct")
-
-    def __init__(self, x, v, F, C, body_pos, body_rot):
-        self.x, self.v, self.F, self.C, self.body_pos, self.body_rot = x, v, F, C, body_pos, body_rot
-        self._ct = _lib.DdmState(*(_lib.ptr(t) for t in (x, v, F, C, body_pos, body_rot)))
-
-    def ctype(self):
-        return self._ct
 
 
 class _Particles:
@@ -81,23 +67,15 @@ class Rollout:
         self.seg = [_Particles(P, dev) for _ in range(self.K + 1)]
         self.g = [_Particles(P, dev, identity=False) for _ in range(2)]
         pp, pr, gp, gr = self.pose_pos, self.pose_rot, self.pos_grad, self.rot_grad
-        # state view of segment slot j at absolute substep t
-        self._sv = {}
-        self._gv = {}
 
+        # per-scene state tables: segment slot j with the poses of absolute substep t
         def sview(j, t):
-            key = (j, t)
-            if key not in self._sv:
-                s = self.seg[j]
-                self._sv[key] = _View(s.x, s.v, s.F, s.C, pp[t], pr[t])
-            return self._sv[key]
+            s = self.seg[j]
+            return engine.table(s.x, s.v, s.F, s.C, pp[t], pr[t])
 
         def gview(j, t):
-            key = (j, t)
-            if key not in self._gv:
-                s = self.g[j]
-                self._gv[key] = _View(s.x, s.v, s.F, s.C, gp[t], gr[t])
-            return self._gv[key]
+            s = self.g[j]
+            return engine.table(s.x, s.v, s.F, s.C, gp[t], gr[t])
 
         self._sview, self._gview = sview, gview
 
@@ -125,10 +103,7 @@ class Rollout:
 
     # ------------------------------------------------------------------ passes
     def _fwd(self, j, t):
-        eng = self.engine
-        _lib.check(eng.lib.ddm_substep_forward(ctypes.byref(eng._batch), ctypes.byref(self._sview(j, t).ctype()),
-                                               ctypes.byref(self._sview(j + 1, t + 1).ctype()),
-                                               _lib.stream_ptr()), "ddm_substep_forward")
+        self.engine.forward_tables(self._sview(j, t), self._sview(j + 1, t + 1))
 
     def forward(self):
         """T forward substeps from the initial state."""
@@ -169,10 +144,8 @@ class Rollout:
                 nxt_slot = cur_slot
                 cur_slot = 1 - cur_slot
                 self.g[cur_slot].zero_()
-                _lib.check(eng.lib.ddm_substep_backward(
-                    ctypes.byref(eng._batch), ctypes.byref(self._sview(j, t).ctype()),
-                    ctypes.byref(self._sview(j + 1, t + 1).ctype()), ctypes.byref(self._gview(cur_slot, t).ctype()),
-                    ctypes.byref(self._gview(nxt_slot, t + 1).ctype()), _lib.stream_ptr()), "ddm_substep_backward")
+                eng.backward_tables(self._sview(j, t), self._sview(j + 1, t + 1), self._gview(cur_slot, t),
+                                    self._gview(nxt_slot, t + 1))
         self._g0 = self.g[cur_slot]
         return self._g0
 

@@ -1,0 +1,414 @@
+"""Continuous-batching server for differentiable MPM rollouts.
+
+Each request is a trajectory-optimisation rollout of one scene: ``T`` forward substeps
+with given per-substep primitive poses, an L2 loss of the final particle positions to a
+target, and the adjoint pass producing d loss / d poses and d loss / d initial state.
+Requests of different particle count, grid size and horizon share the GPU:
+
+* admission: a request is admitted when its grid blocks fit in the grid arena and its
+  device memory (states, checkpoints, temporaries) fits the memory budget; until then
+  it waits in FIFO order. Slots and memory are returned when a request finishes, and
+  waiting requests are admitted at the next tick (continuous batching, not a queue in
+  front of a one-at-a-time executor).
+* execution: every tick advances every admitted request by one substep operation.
+  All forward-type operations (forward substeps and recomputation of checkpointed
+  segments) run in one fused launch sequence over all those scenes, and all adjoint
+  substeps run in one fused adjoint launch sequence (``ddm_substep_forward`` /
+  ``ddm_substep_backward`` with per-scene state tables).
+* state stays on the device for the whole request; only the small scene/chunk/state
+  tables are uploaded per tick.
+
+Latency of a request = completion (an event recorded after its last operation) minus
+arrival; throughput = particle-substeps (forward + adjoint) per second of wall time.
+"""
+from __future__ import annotations
+
+import heapq
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .engine import CHUNK, SceneSpec, chunk_table, state_table
+
+
+@dataclass
+class Request:
+    rid: int
+    spec: SceneSpec
+    x0: np.ndarray               # [n, 3]
+    pose_pos: np.ndarray         # [T+1, nb, 3]
+    pose_rot: np.ndarray         # [T+1, nb, 4]
+    target: np.ndarray           # [n, 3]
+    arrival: float = 0.0         # seconds after the start of the stream
+    checkpoint_every: Optional[int] = None
+
+    @property
+    def T(self) -> int:
+        return int(len(self.pose_pos) - 1)
+
+    @property
+    def n(self) -> int:
+        return int(len(self.x0))
+
+
+@dataclass
+class Result:
+    rid: int
+    loss: float
+    x_final: torch.Tensor
+    pos_grad: torch.Tensor
+    rot_grad: torch.Tensor
+    x0_grad: torch.Tensor
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class BlockArena:
+    """First-fit allocator of contiguous node-block ranges in the grid arena."""
+
+    def __init__(self, n_blocks: int):
+        self.n = n_blocks
+        self.free = [(0, n_blocks)]  # sorted (start, length)
+
+    def alloc(self, k: int) -> Optional[int]:
+        for i, (s, l) in enumerate(self.free):
+            if l >= k:
+                if l == k:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + k, l - k)
+                return s
+        return None
+
+    def release(self, s: int, k: int):
+        self.free.append((s, k))
+        self.free.sort()
+        merged = []
+        for a, l in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + l)
+            else:
+                merged.append((a, l))
+        self.free = merged
+
+    @property
+    def free_blocks(self) -> int:
+        return sum(l for _, l in self.free)
+
+
+class _Particles:
+    def __init__(self, n, device, identity=True):
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=device)
+        self.x, self.v, self.F, self.C = z(n, 3), z(n, 3), z(n, 3, 3), z(n, 3, 3)
+        if identity and n:
+            self.F[:] = torch.eye(3, device=device)
+
+    def tensors(self):
+        return (self.x, self.v, self.F, self.C)
+
+    def copy_(self, o):
+        for a, b in zip(self.tensors(), o.tensors()):
+            a.copy_(b)
+
+    def zero_(self):
+        for a in self.tensors():
+            a.zero_()
+
+
+def request_bytes(req: Request) -> int:
+    """Device bytes a request holds while active."""
+    T = req.T
+    K = req.checkpoint_every or T
+    n_states = (T // K + 1 if K != T else 0) + (K + 1) + 2
+    per_state = req.n * 24 * 4
+    temps = req.n * (9 * 3 + 3 + 5) * 4
+    nb = max(req.spec.n_bodies, 1)
+    poses = (T + 1) * nb * 7 * 4 * 2
+    gd = req.spec.grid_dim
+    blocks = (gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4)
+    return n_states * per_state + temps + poses + blocks * 64 * 16 * 4
+
+
+class _Active:
+    """Device-side state and operation schedule of an admitted request."""
+
+    def __init__(self, server: "RolloutServer", req: Request, blk_begin: int, now: float):
+        dev = server.device
+        self.req = req
+        self.admitted = now
+        sp = req.spec
+        self.blk_begin = blk_begin
+        gd = [int(g) for g in sp.grid_dim]
+        self.n_blocks = (gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4)
+        n, nb, T = req.n, sp.n_bodies, req.T
+        self.K = req.checkpoint_every or T
+        if T % self.K:
+            raise ValueError("checkpoint_every must divide the horizon")
+        f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+        self.mass, self.vol, self.mly = f(sp.mass), f(sp.vol), f(np.asarray(sp.mu_lam_yield).reshape(-1, 3))
+        self.prim = f(np.asarray(sp.body_prim).reshape(-1, 4)) if nb else None
+        self.args = f(np.asarray(sp.body_args).reshape(-1, 4)) if nb else None
+        self.tmp = [torch.zeros(n, 9, device=dev), torch.zeros(n, 9, device=dev), torch.zeros(n, 9, device=dev),
+                    torch.zeros(n, 3, device=dev)]
+        nb_rows = max(nb, 1)
+        self.pose_pos = torch.zeros(T + 1, nb_rows, 3, device=dev)
+        self.pose_rot = torch.zeros(T + 1, nb_rows, 4, device=dev)
+        self.pose_rot[..., 0] = 1
+        if nb:
+            self.pose_pos.copy_(f(req.pose_pos))
+            self.pose_rot.copy_(f(req.pose_rot))
+        self.pos_grad = torch.zeros_like(self.pose_pos)
+        self.rot_grad = torch.zeros_like(self.pose_rot)
+        self.full = self.K == T
+        self.ckpt = [] if self.full else [_Particles(n, dev) for _ in range(T // self.K + 1)]
+        self.seg = [_Particles(n, dev) for _ in range(self.K + 1)]
+        self.g = [_Particles(n, dev, identity=False) for _ in range(2)]
+        self.seg[0].x.copy_(f(req.x0))
+        if not self.full:
+            self.ckpt[0].copy_(self.seg[0])
+        self.target = f(req.target)
+        self.record = np.zeros(1, dtype=_lib.SCENE_DTYPE)[0]
+        r = self.record
+        r["p_count"] = n
+        r["blk_begin"] = blk_begin
+        r["n_blocks"] = self.n_blocks
+        r["dim"] = gd
+        r["n_bodies"] = nb
+        r["adjoint_mode"] = int(sp.adjoint_mode)
+        r["dx"], r["inv_dx"], r["dt"] = sp.dx, 1.0 / sp.dx, sp.dt
+        r["ground_friction"], r["ground_height"] = sp.ground_friction, sp.ground_height
+        r["gravity"] = np.asarray(sp.gravity, np.float32)
+        r["mass"], r["vol"], r["mu_lam_yield"] = self.mass.data_ptr(), self.vol.data_ptr(), self.mly.data_ptr()
+        r["body_prim"] = self.prim.data_ptr() if nb else 0
+        r["body_args"] = self.args.data_ptr() if nb else 0
+        r["tmp_Ft"], r["tmp_U"], r["tmp_V"], r["tmp_sig"] = (t.data_ptr() for t in self.tmp)
+        self.chunks = chunk_table([n])[:, 1:3]  # local (first, end) rows
+        self.ops = self._schedule()
+        self.pc = 0
+        self.done_event = None
+        self.loss = None
+
+    # ---- operation schedule: ("pre", callable) run before the tick's launches,
+    #      ("F", j, t) forward substep seg[j] -> seg[j+1] at substep t,
+    #      ("B", j, t, gcur, gnxt) adjoint substep.
+    def _schedule(self):
+        T, K = self.req.T, self.K
+        ops: List[tuple] = []
+        nseg = T // K
+        for c in range(nseg):
+            if not self.full and c > 0:
+                ops.append(("pre", lambda c=c: self.seg[0].copy_(self.ckpt[c]), c))
+            for j in range(K):
+                ops.append(("F", j, c * K + j))
+            if not self.full:
+                ops.append(("post", lambda c=c: self.ckpt[c + 1].copy_(self.seg[K]), c))
+        ops.append(("loss", None, None))
+        cur = 0
+        for c in range(nseg - 1, -1, -1):
+            if not self.full:
+                ops.append(("pre", lambda c=c: self.seg[0].copy_(self.ckpt[c]), c))
+                for j in range(K):
+                    ops.append(("F", j, c * K + j))
+            for j in range(K - 1, -1, -1):
+                nxt, cur = cur, 1 - cur
+                ops.append(("B", j, c * K + j, cur, nxt))
+        self.final_grad_slot = cur
+        return ops
+
+    def _final(self):
+        return self.seg[self.K] if self.full else self.ckpt[-1]
+
+    def loss_and_seed(self):
+        x = self._final().x
+        d = x - self.target
+        self.loss = (d * d).sum(-1).mean()
+        g = self.g[0]
+        g.zero_()
+        g.x.copy_(2.0 / self.req.n * d)
+        self.pos_grad.zero_()
+        self.rot_grad.zero_()
+        self.x_final = x.clone()
+
+    def state_record(self, j, t):
+        s = self.seg[j]
+        return (s.x.data_ptr(), s.v.data_ptr(), s.F.data_ptr(), s.C.data_ptr(), self.pose_pos[t].data_ptr(),
+                self.pose_rot[t].data_ptr())
+
+    def grad_record(self, slot, t):
+        g = self.g[slot]
+        return (g.x.data_ptr(), g.v.data_ptr(), g.F.data_ptr(), g.C.data_ptr(), self.pos_grad[t].data_ptr(),
+                self.rot_grad[t].data_ptr())
+
+    def next_kernel_op(self):
+        """Run host-side pre/post operations up to the next kernel operation and return
+        it (or None when the schedule is finished)."""
+        while self.pc < len(self.ops):
+            op = self.ops[self.pc]
+            if op[0] in ("pre", "post"):
+                op[1]()
+                self.pc += 1
+            elif op[0] == "loss":
+                self.loss_and_seed()
+                self.pc += 1
+            else:
+                return op
+        return None
+
+
+class RolloutServer:
+    def __init__(self, device="cuda:0", arena_blocks: int = 64 * 13824, memory_budget: int = 64 << 30,
+                 max_active: int = 256):
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        self.arena = BlockArena(arena_blocks)
+        G = arena_blocks * 64
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=self.device)
+        self.grid_mmv, self.grid_v, self.grid_dv, self.grid_dmmv = z(G, 4), z(G, 4), z(G, 4), z(G, 4)
+        self.block_flags = torch.zeros(arena_blocks, dtype=torch.int32, device=self.device)
+        self.memory_budget = memory_budget
+        self.max_active = max_active
+        self.waiting: List[Request] = []
+        self.active: List[_Active] = []
+        self.results: Dict[int, Result] = {}
+        self.used_bytes = 0
+        self.t0 = None
+        self.ticks = 0
+        self.particle_substeps = 0
+
+    # ------------------------------------------------------------------ admission
+    def submit(self, req: Request):
+        if request_bytes(req) > self.memory_budget:
+            raise ValueError(f"request {req.rid} needs more memory than the whole budget")
+        gd = req.spec.grid_dim
+        if (gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4) > self.arena.n:
+            raise ValueError(f"request {req.rid} grid exceeds the arena")
+        req.spec.validate()
+        self.waiting.append(req)
+
+    def _now(self):
+        return time.perf_counter() - self.t0
+
+    def _admit(self, now):
+        while self.waiting and len(self.active) < self.max_active:
+            req = self.waiting[0]
+            if req.arrival > now:
+                break
+            need = request_bytes(req)
+            if self.used_bytes + need > self.memory_budget:
+                break
+            gd = req.spec.grid_dim
+            nblk = (gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4)
+            b0 = self.arena.alloc(nblk)
+            if b0 is None:
+                break
+            self.waiting.pop(0)
+            self.used_bytes += need
+            self.active.append(_Active(self, req, b0, now))
+
+    # ------------------------------------------------------------------ launches
+    def _launch(self, entries, kind):
+        """entries: list of (active, op). One fused launch sequence for all of them."""
+        entries.sort(key=lambda e: e[0].blk_begin)  # the grid kernels look scenes up by block
+        recs = np.stack([e[0].record for e in entries])
+        rows = []
+        cur, nxt, gc, gn = [], [], [], []
+        for si, (a, op) in enumerate(entries):
+            for (b, e) in a.chunks:
+                rows.append((si, int(b), int(e), 0))
+            j, t = op[1], op[2]
+            cur.append(a.state_record(j, t))
+            nxt.append(a.state_record(j + 1, t + 1))
+            if kind == "B":
+                gc.append(a.grad_record(op[3], t))
+                gn.append(a.grad_record(op[4], t + 1))
+        dev = self.device
+        scene_tab = torch.as_tensor(recs.view(np.uint8).copy(), device=dev)
+        chunks = torch.as_tensor(np.asarray(rows, np.int32), device=dev)
+        tabs = [state_table(x, dev) for x in ((cur, nxt, gc, gn) if kind == "B" else (cur, nxt))]
+        batch = _lib.DdmBatch(len(entries), len(rows), self.arena.n, max(int(r["n_bodies"]) for r in recs),
+                              scene_tab.data_ptr(), chunks.data_ptr(), self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
+                              self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr())
+        import ctypes
+        st = _lib.stream_ptr()
+        if kind == "F":
+            _lib.check(self.lib.ddm_substep_forward(ctypes.byref(batch), tabs[0].data_ptr(), tabs[1].data_ptr(), st),
+                       "ddm_substep_forward")
+        else:
+            _lib.check(self.lib.ddm_substep_backward(ctypes.byref(batch), tabs[0].data_ptr(), tabs[1].data_ptr(),
+                                                     tabs[2].data_ptr(), tabs[3].data_ptr(), st),
+                       "ddm_substep_backward")
+        # keep the tables alive until the stream has consumed them
+        self._keep.append((scene_tab, chunks, tabs))
+        for a, _ in entries:
+            self.particle_substeps += a.req.n if kind == "B" else 0
+
+    def tick(self):
+        now = self._now()
+        self._admit(now)
+        fwd, bwd = [], []
+        finished = []
+        for a in self.active:
+            if a.done_event is not None:
+                continue
+            op = a.next_kernel_op()
+            if op is None:
+                a.done_event = torch.cuda.Event()
+                a.done_event.record()
+                finished.append(a)
+                continue
+            (fwd if op[0] == "F" else bwd).append((a, op))
+        self._keep = []
+        if fwd:
+            self._launch(fwd, "F")
+        if bwd:
+            self._launch(bwd, "B")
+        for a, _ in fwd + bwd:
+            a.pc += 1
+        self.ticks += 1
+        self._collect()
+
+    def _collect(self, block=False):
+        still = []
+        for a in self.active:
+            if a.done_event is not None and (block or a.done_event.query()):
+                a.done_event.synchronize()
+                fin = self._now()
+                g0 = a.g[a.final_grad_slot]
+                self.results[a.req.rid] = Result(a.req.rid, float(a.loss.item()), a.x_final, a.pos_grad, a.rot_grad,
+                                                 g0.x.clone(), a.req.arrival, a.admitted, fin)
+                self.used_bytes -= request_bytes(a.req)
+                self.arena.release(a.blk_begin, a.n_blocks)
+            else:
+                still.append(a)
+        self.active = still
+
+    def run(self, requests: List[Request], timeout: float = 600.0) -> Dict[int, Result]:
+        """Serve a stream of requests (arrival times relative to the call) until all
+        have finished."""
+        for r in sorted(requests, key=lambda r: r.arrival):
+            self.submit(r)
+        self.t0 = time.perf_counter()
+        self._keep = []
+        while self.waiting or self.active:
+            if not self.active and self.waiting and self.waiting[0].arrival > self._now():
+                time.sleep(min(1e-3, self.waiting[0].arrival - self._now()))
+            self.tick()
+            if self._now() > timeout:
+                raise TimeoutError("rollout server did not drain in time")
+        torch.cuda.synchronize()
+        return self.results
+
+
+def latency_stats(results: Dict[int, Result]):
+    lat = np.array([r.latency for r in results.values()]) * 1e3
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "n": int(len(lat))}

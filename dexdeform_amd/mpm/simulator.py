@@ -487,7 +487,10 @@ class MPMSimulator:
         if self.torch_scale is None or self.torch_scale.device != torch.device(device):
             self.torch_scale = torch.tensor(np.array(self.action_scales), device=device, dtype=torch.float32)
         pos, rot = primitive_poses(pos, rot, action, self.torch_scale, self.substeps)
-        return pos, rot, (pos, rot)
+        # the carried kinematic state is the pose after the last substep (the reference
+        # returns the whole trajectory here, mpm/simulator.py:624, which GradModel.forward
+        # cannot feed back in for the next env step)
+        return pos, rot, (pos[-1], rot[-1])
 
     def step(self, action, pos_rot=None):
         pos, rot, q_state = self.compute_forward_kinematics(self.cur, action, pos_rot=pos_rot)

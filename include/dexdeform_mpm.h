@@ -100,55 +100,59 @@ int particle2mass(float *particle_x, float *particle_m, int *grid_lower, const i
                   int *particle_ids, int id, int compute_grad, int dim, hipStream_t stream);
 
 /* ---------------------------------------------------------------- group 2
- * Multi-scene fused substep. All pointers in the structs are device pointers unless
- * noted. Particles of all scenes are concatenated; grids are per scene, 4x4x4-blocked. */
+ * Multi-scene fused substep. Scenes are independent: each has its own particle
+ * arrays, material, primitives and poses, addressed through per-scene records in
+ * device memory, so a launch can mix scenes of different sizes, grids and rollout
+ * phases. Grid work arrays are shared arenas of 4x4x4 node blocks; each scene owns
+ * n_blocks consecutive blocks starting at blk_begin. */
 
 #define DDM_MAX_BODIES 64
 
 typedef struct ddm_scene {
-  int32_t p_begin, p_count;   /* particle range */
-  int32_t blk_begin, n_blocks;/* first 4^3 node block (grid arrays hold 64 nodes per block) */
-  int32_t dim[3];             /* grid nodes per axis, multiples of 4 */
-  int32_t n_bodies;
-  int32_t b_begin;            /* first body in the body arrays */
-  int32_t adjoint_mode;       /* 0 = reference adjoint, 1 = exact (finite-difference consistent) */
-  int32_t pad0, pad1;
+  int32_t p_count;             /* particles of the scene (chunks define the work) */
+  int32_t blk_begin, n_blocks; /* node blocks of the scene in the grid arenas */
+  int32_t dim[3];              /* grid nodes per axis, multiples of 4 */
+  int32_t n_bodies;            /* <= DDM_MAX_BODIES */
+  int32_t adjoint_mode;        /* 0 = reference adjoint, 1 = exact (finite-difference consistent) */
   float dx, inv_dx, dt, ground_friction;
-  float ground_height;        /* in cells (3 in the reference, mpm/simulator.py:493-494) */
-  float gravity[3];           /* as seen by the kernel: configured gravity * 30 (mpm/simulator.py:385) */
-} ddm_scene;                  /* 80 bytes */
+  float ground_height;         /* in cells (3 in the reference, mpm/simulator.py:493-494) */
+  float gravity[3];            /* as seen by the kernel: configured gravity * 30 (mpm/simulator.py:385) */
+  const float *mass, *vol;     /* [p_count] */
+  const float *mu_lam_yield;   /* [p_count][3] */
+  const float *body_prim;      /* [n_bodies][4] (type, friction, softness, round) */
+  const float *body_args;      /* [n_bodies][4] */
+  float *tmp_Ft, *tmp_U, *tmp_V, *tmp_sig; /* [p_count][9],[..][9],[..][9],[..][3]; backward only */
+} ddm_scene;                   /* 136 bytes */
 
 typedef struct ddm_batch {
   int32_t n_scenes;
-  int32_t n_chunks;           /* particle chunks of <= 256 particles, none straddling scenes */
-  int32_t n_blocks;           /* total node blocks over all scenes */
-  int32_t max_bodies;         /* max n_bodies over scenes */
-  const ddm_scene *scenes;    /* [n_scenes] */
-  const int32_t *chunks;      /* [n_chunks][4] = (scene, p_begin, p_end, 0) */
-  const float *mass, *vol;    /* [P] */
-  const float *mu_lam_yield;  /* [P][3] */
-  const float *body_prim;     /* [NB][4] (type, friction, softness, round) */
-  const float *body_args;     /* [NB][4] */
-  float *grid_mmv;            /* [64*n_blocks][4] (m, mv) ; zero between substeps */
-  float *grid_v;              /* [64*n_blocks][4] node velocity after the grid update */
-  float *grid_dv;             /* [64*n_blocks][4] adjoint of grid_v ; zero between substeps */
-  float *grid_dmmv;           /* [64*n_blocks][4] adjoint of (m, mv) */
-  uint32_t *block_flags;      /* [n_blocks] ; zero between substeps */
-  float *tmp_Ft, *tmp_U, *tmp_V, *tmp_sig; /* [P][9],[P][9],[P][9],[P][3] ; backward only */
+  int32_t n_chunks;            /* particle chunks of <= 256 particles, none straddling scenes */
+  int32_t n_blocks;            /* node blocks in the grid arenas */
+  int32_t max_bodies;          /* max n_bodies over the scenes */
+  const ddm_scene *scenes;     /* device [n_scenes], sorted by blk_begin */
+  const int32_t *chunks;       /* device [n_chunks][4] = (scene, first, end, 0), local particle indices */
+  float *grid_mmv;             /* [64*n_blocks][4] (m, mv); zero between substeps */
+  float *grid_v;               /* [64*n_blocks][4] node velocity after the grid update */
+  float *grid_dv;              /* [64*n_blocks][4] adjoint of grid_v; zero between substeps */
+  float *grid_dmmv;            /* [64*n_blocks][4] adjoint of (m, mv) */
+  uint32_t *block_flags;       /* [n_blocks]; zero between substeps */
 } ddm_batch;
 
+/* Per-scene state record (device memory, one per scene, in scene order). */
 typedef struct ddm_state {
-  float *x, *v, *F, *C;       /* [P][3], [P][3], [P][9], [P][9] */
-  float *body_pos, *body_rot; /* [NB][3], [NB][4] */
+  float *x, *v, *F, *C;        /* [p_count][3], [..][3], [..][9], [..][9] */
+  float *body_pos, *body_rot;  /* [n_bodies][3], [n_bodies][4] */
 } ddm_state;
 
-/* One forward substep of every scene: cur -> next (next->body_* must hold the poses of
- * substep f+1). Writes next->x, v, F, C. Replaces MPMSimulator.substep. */
+/* One forward substep of every scene: cur[s] -> next[s] (next[s].body_* must hold the
+ * poses of substep f+1). Writes next x, v, F, C. cur and next are DEVICE arrays of
+ * n_scenes records. Replaces MPMSimulator.substep. */
 int ddm_substep_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream);
 
-/* Adjoint of one substep of every scene. Reads cur, next (state and poses) and
- * next_grad->x, v, F, C; ADDS into cur_grad->x, v, F, C, body_pos, body_rot and into
- * next_grad->body_pos, body_rot. Replaces MPMSimulator.substep_grad. */
+/* Adjoint of one substep of every scene. Reads cur, next (state and poses) and the
+ * next_grad x, v, F, C; ADDS into the cur_grad x, v, F, C, body_pos, body_rot and into
+ * next_grad body_pos, body_rot. All four are DEVICE arrays of n_scenes records.
+ * Replaces MPMSimulator.substep_grad. */
 int ddm_substep_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                          const ddm_state *next_grad, hipStream_t stream);
 
@@ -163,7 +167,7 @@ int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur, const d
                                const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
                                hipEvent_t *events);
 
-/* Zeroes the batch's grid work arrays and block flags (after an error or on setup). */
+/* Zeroes the batch's grid arenas and block flags (after an error or on setup). */
 int ddm_batch_reset(const ddm_batch *b, hipStream_t stream);
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -36,6 +36,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     import ctypes
     from dexdeform_amd import _lib
-    assert ctypes.sizeof(_lib.DdmScene) == 80
+    assert ctypes.sizeof(_lib.DdmScene) == 136
     assert ctypes.sizeof(_lib.DdmState) == 6 * 8
-    assert ctypes.sizeof(_lib.DdmBatch) == 16 + 16 * 8
+    assert ctypes.sizeof(_lib.DdmBatch) == 16 + 7 * 8
+    hdr = open(HEADER).read()
+    assert "} ddm_scene;                   /* 136 bytes */" in hdr
