@@ -60,18 +60,59 @@ struct Tile {
   __device__ int local(int x, int y, int z) const { return ((x - lo[0]) * t[1] + (y - lo[1])) * t[2] + (z - lo[2]); }
 };
 
-// bounding box of the chunk's stencil bases -> tile covering all stencil nodes
+// Scatter strategy (compile-time):
+//  DDM_SCATTER_ROTATE = 1: each lane visits the 27 stencil nodes starting at a
+//    lane-dependent offset, so lanes whose particles share a cell hit distinct LDS
+//    addresses in each instruction; 0: all lanes visit in the reference's order with the
+//    loops fully unrolled (cheaper address and weight arithmetic).
+//  DDM_INTERLEAVE = 1: lane l of wave w handles particle 4*l + w of its 256-particle
+//    chunk, so particles adjacent in the (spatially sorted) order land in different
+//    waves; 0: lane = particle.
+#ifndef DDM_SCATTER_ROTATE
+#define DDM_SCATTER_ROTATE 1
+#endif
+#ifndef DDM_INTERLEAVE
+#define DDM_INTERLEAVE 0
+#endif
+
+__device__ __forceinline__ int chunk_slot(int t) {
+#if DDM_INTERLEAVE
+  return ((t & 63) << 2) | (t >> 6);
+#else
+  return t;
+#endif
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// bounding box of the chunk's stencil bases -> tile covering all stencil nodes.
+// Reduced across each wavefront first, then one LDS atomic per wave and bound.
 __device__ Tile chunk_tile(bool valid, const int base[3], int *s_box) {
   if (threadIdx.x < 3) {
     s_box[threadIdx.x] = INT_MAX;
     s_box[3 + threadIdx.x] = INT_MIN;
   }
   __syncthreads();
-  if (valid) {
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = wave_min(valid ? base[d] : INT_MAX);
+    hi[d] = wave_max(valid ? base[d] : INT_MIN);
+  }
+  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      atomicMin(&s_box[d], base[d]);
-      atomicMax(&s_box[3 + d], base[d]);
+      atomicMin(&s_box[d], lo[d]);
+      atomicMax(&s_box[3 + d], hi[d]);
     }
   }
   __syncthreads();
@@ -119,7 +160,7 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x];
-  const int p = ch.y + (int)threadIdx.x;
+  const int p = ch.y + chunk_slot((int)threadIdx.x);
   const bool valid = p < ch.z;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   P2GParticle q;
@@ -149,8 +190,7 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
     for (int i = threadIdx.x; i < T.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
     __syncthreads();
     if (valid) {
-      // lanes start at different stencil offsets: particles sharing a cell (adjacent
-      // in the sorted order) then hit distinct LDS addresses in each instruction
+#if DDM_SCATTER_ROTATE
       const int start = (int)(threadIdx.x % 27u);
 #pragma unroll 1
       for (int oo = 0; oo < 27; ++oo) {
@@ -158,6 +198,14 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
         o = o >= 27 ? o - 27 : o;
         int i, j, k;
         offset_ijk(o, i, j, k);
+#else
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+#endif
         float cm;
         f3 cmv;
         p2g_contrib(q, S.dx, i, j, k, cm, cmv);
@@ -381,7 +429,7 @@ __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__res
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
-  const int p = ch.y + (int)threadIdx.x;
+  const int p = ch.y + chunk_slot((int)threadIdx.x);
   if (p >= ch.z) return;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
@@ -425,7 +473,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
   const ddm_state cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
-  const int p = ch.y + (int)threadIdx.x;
+  const int p = ch.y + chunk_slot((int)threadIdx.x);
   const bool valid = p < ch.z;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   f3 x{0.f, 0.f, 0.f};
@@ -444,7 +492,8 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
       TileScatter3 sc{tx, ty, tz, &T};
       f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
                                 ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc, S.inv_dx,
-                                S.dx, S.dt, S.ground_height, dim, (int)(threadIdx.x % 27u));
+                                S.dx, S.dt, S.ground_height, dim,
+                                DDM_SCATTER_ROTATE ? (int)(threadIdx.x % 27u) : -1);
       add3(cur_grad.x + 3 * (size_t)p, gxp);
     }
     __syncthreads();
@@ -474,7 +523,7 @@ __global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, const ddm_state *_
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
-  const int p = ch.y + (int)threadIdx.x;
+  const int p = ch.y + chunk_slot((int)threadIdx.x);
   if (p >= ch.z) return;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   const size_t p3 = 3 * (size_t)p, p9 = 9 * (size_t)p;

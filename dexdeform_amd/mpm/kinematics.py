@@ -22,8 +22,9 @@ def axis_angle_quat(rot: torch.Tensor) -> torch.Tensor:
 
 
 def quat_mul(q: torch.Tensor, r: torch.Tensor) -> torch.Tensor:
-    """Hamilton product q * r, written as the reference's outer-product terms
-    (mpm/simulator.py:35-41)."""
+    """The reference's quaternion composition from outer-product terms
+    (mpm/simulator.py:35-41): the Hamilton product r * q, i.e. the increment r applied
+    in the world frame after the orientation q."""
     t = q[..., :, None] * r[..., None, :]
     w = t[..., 0, 0] - t[..., 1, 1] - t[..., 2, 2] - t[..., 3, 3]
     x = t[..., 0, 1] + t[..., 1, 0] - t[..., 2, 3] + t[..., 3, 2]

@@ -318,7 +318,7 @@ class RolloutServer:
     def _launch(self, entries, kind):
         """entries: list of (active, op). One fused launch sequence for all of them."""
         entries.sort(key=lambda e: e[0].blk_begin)  # the grid kernels look scenes up by block
-        recs = np.stack([e[0].record for e in entries])
+        recs = np.array([e[0].record for e in entries], dtype=_lib.SCENE_DTYPE)
         rows = []
         cur, nxt, gc, gn = [], [], [], []
         for si, (a, op) in enumerate(entries):
@@ -370,6 +370,8 @@ class RolloutServer:
         if fwd:
             self._launch(fwd, "F")
         if bwd:
+            for a, op in bwd:
+                a.g[op[3]].zero_()  # the adjoint substep adds into the state-t gradient
             self._launch(bwd, "B")
         for a, _ in fwd + bwd:
             a.pc += 1

@@ -1,0 +1,153 @@
+"""Host-side logic that runs without a GPU: kinematics (torch, CPU), chunk and state
+tables, the rollout server's block arena and memory accounting, workload generators."""
+import numpy as np
+import pytest
+import torch
+
+from dexdeform_amd.mpm import kinematics as K
+from dexdeform_amd.mpm import workloads as W
+from dexdeform_amd.mpm.engine import CHUNK, chunk_table, lame
+
+
+def _rand_quat(rng, n):
+    q = rng.normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def test_quat_mul_matches_scipy_composition():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(0)
+    q, r = _rand_quat(rng, 20), _rand_quat(rng, 20)
+    got = K.quat_mul(torch.tensor(q), torch.tensor(r)).numpy()
+    Rq = Rotation.from_quat(q[:, [1, 2, 3, 0]])
+    Rr = Rotation.from_quat(r[:, [1, 2, 3, 0]])
+    exp = (Rr * Rq).as_quat()[:, [3, 0, 1, 2]]  # world-frame increment r after q
+    sign = np.sign((got * exp).sum(1, keepdims=True))
+    assert np.allclose(got, exp * sign, atol=1e-12)
+
+
+def test_quaternion_matrix_round_trip_and_scipy():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(1)
+    q = _rand_quat(rng, 50)
+    M = K.quaternion_to_matrix(torch.tensor(q)).numpy()
+    assert np.allclose(M, Rotation.from_quat(q[:, [1, 2, 3, 0]]).as_matrix(), atol=1e-12)
+    q2 = K.matrix_to_quaternion(torch.tensor(M)).numpy()
+    assert np.allclose(np.abs((q2 * q).sum(1)), 1.0, atol=1e-12)  # same rotation up to sign
+
+
+def test_axis_angle_matches_scipy_rotvec():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(2)
+    aa = rng.normal(size=(40, 3))
+    aa[:5] *= 1e-8  # small-angle branch
+    M = K.axis_angle_to_matrix(torch.tensor(aa)).numpy()
+    assert np.allclose(M, Rotation.from_rotvec(aa).as_matrix(), atol=1e-12)
+
+
+def test_rigid_body_motion_composes_rotation():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(3)
+    B, T = 3, 4
+    pos = torch.tensor(rng.normal(size=(B, 3)))
+    q = torch.tensor(_rand_quat(rng, B))
+    act = torch.tensor(rng.normal(size=(T, B, 6)) * 0.3)
+    p2, r2 = K.rigid_body_motion((pos, q), act)
+    assert np.allclose(p2.numpy(), (pos[None] + act[..., :3]).numpy())
+    exp = (Rotation.from_rotvec(act[..., 3:].reshape(-1, 3).numpy()) *
+           Rotation.from_quat(q.numpy()[None].repeat(T, 0).reshape(-1, 4)[:, [1, 2, 3, 0]])).as_quat()[:, [3, 0, 1, 2]]
+    got = r2.reshape(-1, 4).numpy()
+    assert np.allclose(np.abs((got * exp).sum(1)), 1.0, atol=1e-10)
+
+
+def test_primitive_poses_ramp():
+    pos = torch.zeros(2, 3)
+    rot = torch.tensor([[1.0, 0, 0, 0]] * 2)
+    a = torch.tensor([[2.0, -0.5, 0.25, 0, 0, 0], [0.1, 0.0, 0.0, 0, 0, 0]])
+    scales = torch.tensor([[1e-3] * 6] * 2)
+    p, r = K.primitive_poses(pos, rot, a, scales, 10)
+    # clamp to [-1, 1], then substep i moves a * (i + 1) / substeps
+    assert torch.allclose(p[-1, 0], torch.tensor([1e-3, -0.5e-3, 0.25e-3]))
+    assert torch.allclose(p[0, 1], torch.tensor([0.1e-4, 0, 0]))
+    assert torch.allclose(r, rot.expand_as(r))
+
+
+def test_hand_rigid_body_motion_translation_and_unit_rotation():
+    rng = np.random.default_rng(4)
+    base = torch.eye(4)[None].repeat(2, 1, 1)
+    base[:, :3, :3] = K.quaternion_to_matrix(torch.tensor(_rand_quat(rng, 2), dtype=torch.float32))
+    acts = torch.tensor(rng.normal(size=(2, 6)) * 0.01, dtype=torch.float32)
+    out = K.rigid_body_motion_hand(base, acts, 5)
+    assert out.shape == (5, 2, 4, 4)
+    assert torch.allclose(out[-1, :, :3, 3], base[:, :3, 3] + acts[:, :3], atol=1e-6)
+    R = out[..., :3, :3]
+    assert torch.allclose(R @ R.transpose(-1, -2), torch.eye(3).expand_as(R), atol=1e-5)
+
+
+def test_hinge_tree_kinematics_and_gradients():
+    tree, args, q0 = K.synthetic_hand()
+    assert args.shape == (22, 4) and np.all((args[:, 0] >= 0.008) & (args[:, 0] <= 0.015))
+    a = torch.zeros(22, requires_grad=True)
+    pos, rot, qe = tree.poses(q0, a, 8)
+    assert pos.shape == (8, 22, 3) and rot.shape == (8, 22, 4)
+    assert torch.allclose(torch.linalg.norm(rot, dim=-1), torch.ones(8, 22), atol=1e-5)
+    # joint velocity ramp without /substeps (mpm/hand.py:414-415), clamped to limits
+    a2 = torch.ones(22)
+    q = tree.joint_targets(q0, a2, 8)
+    assert torch.allclose(q[3], torch.clamp(q0 + 0.33 * 0.002 * 4, -0.6, 0.6))
+    (pos.sum() + rot.sum()).backward()
+    assert a.grad is not None and torch.isfinite(a.grad).all()
+
+
+def test_chunk_table_covers_every_particle_once():
+    counts = [1, 255, 256, 257, 1000, 0]
+    tab = chunk_table(counts)
+    assert np.all(tab[:, 2] - tab[:, 1] <= CHUNK) and np.all(tab[:, 2] > tab[:, 1])
+    for s, n in enumerate(counts):
+        rows = tab[tab[:, 0] == s]
+        covered = np.concatenate([np.arange(b, e) for _, b, e, _ in rows]) if len(rows) else np.zeros(0)
+        assert np.array_equal(np.sort(covered), np.arange(n))
+
+
+def test_lame_parameters():
+    mu, lam = lame(5e3, 0.2)
+    assert np.isclose(mu, 5e3 / 2.4) and np.isclose(lam, 5e3 * 0.2 / (1.2 * 0.6))
+
+
+def test_block_arena_alloc_release_merge():
+    from dexdeform_amd.mpm.serve import BlockArena
+    ar = BlockArena(100)
+    a = ar.alloc(30)
+    b = ar.alloc(50)
+    assert (a, b) == (0, 30) and ar.alloc(30) is None
+    ar.release(a, 30)
+    c = ar.alloc(10)
+    assert c == 0
+    ar.release(b, 50)
+    ar.release(c, 10)
+    assert ar.free == [(0, 100)]
+
+
+def test_request_bytes_scales_with_checkpointing():
+    from dexdeform_amd.mpm.serve import Request, request_bytes
+    sc = W.plain_scene(n_particles=2000, grid=32)
+    T = 40
+    pos = np.repeat(sc.body_pos[None], T + 1, 0)
+    rot = np.repeat(sc.body_rot[None], T + 1, 0)
+    full = Request(0, sc.spec, sc.x0, pos, rot, sc.x0)
+    ck = Request(1, sc.spec, sc.x0, pos, rot, sc.x0, checkpoint_every=10)
+    assert request_bytes(ck) < request_bytes(full)
+
+
+def test_workloads_shapes_and_order():
+    sc = W.plain_scene(n_particles=3000)
+    assert sc.x0.shape == (3000, 3) and sc.nb == 2
+    lo, hi = np.array([0.4, 0.05, 0.4]), np.array([0.6, 0.25, 0.6])
+    assert np.all(sc.x0 >= lo) and np.all(sc.x0 <= hi)
+    assert np.isclose(sc.spec.gravity[1], -9.8 * 30)
+    scenes = W.batched_scenes(n_scenes=4, seed=3, n_range=(100, 200))
+    assert len(scenes) == 4 and all(100 <= s.n <= 200 for s in scenes)
+    assert all(2 <= s.nb <= 22 for s in scenes)
+    x = W.ellipsoid_particles(500, [0.5, 0.2, 0.5], [0.12, 0.08, 0.1], seed=3)
+    r = ((x - [0.5, 0.2, 0.5]) / [0.12, 0.08, 0.1]) ** 2
+    assert np.all(r.sum(1) <= 1 + 1e-5)
