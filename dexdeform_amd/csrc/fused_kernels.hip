@@ -11,8 +11,9 @@
 //    all zero: the last grid kernel of a substep clears what it consumed, so no dense
 //    memset is ever needed.
 //  * P2G and the G2P adjoint scatter into an LDS tile covering the bounding box of the
-//    chunk's stencils and flush only the non-zero tile nodes with global atomics; when
-//    the box exceeds the tile budget the chunk scatters with global atomics directly.
+//    chunk's stencils and flush only the non-zero tile nodes with global atomics. A box
+//    larger than the tile is processed in x-slabs, one tile pass per slab; only a box
+//    whose single x-plane exceeds the tile scatters with global atomics directly.
 //  * The contact adjoint reduces body-pose gradients across the wavefront before one
 //    atomic per body per wave.
 #include <limits.h>
@@ -58,7 +59,20 @@ struct BlockedGatherDMMV {
 struct Tile {
   int lo[3], t[3], n;
   __device__ int local(int x, int y, int z) const { return ((x - lo[0]) * t[1] + (y - lo[1])) * t[2] + (z - lo[2]); }
+  __device__ bool has_x(int x) const { return x >= lo[0] && x < lo[0] + t[0]; }
 };
+// nodes per x-plane of the chunk box must fit the tile for the slab passes
+__device__ __forceinline__ bool slabs_fit(const Tile &T) { return T.t[1] * T.t[2] <= kTileMax; }
+// x-planes per slab pass
+__device__ __forceinline__ int slab_width(const Tile &T) { return min(T.t[0], kTileMax / (T.t[1] * T.t[2])); }
+// the slab of the chunk box starting at x-plane xs
+__device__ __forceinline__ Tile slab(const Tile &T, int xs, int sx) {
+  Tile L = T;
+  L.lo[0] = xs;
+  L.t[0] = min(sx, T.lo[0] + T.t[0] - xs);
+  L.n = L.t[0] * T.t[1] * T.t[2];
+  return L;
+}
 
 // Scatter strategy (compile-time):
 //  DDM_SCATTER_ROTATE = 1: each lane visits the 27 stencil nodes starting at a
@@ -186,46 +200,56 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
   }
   Tile T = chunk_tile(valid, q.st.base, s_box);
   float *gmmv = B.grid_mmv;
-  if (T.n <= kTileMax) {
-    for (int i = threadIdx.x; i < T.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
-    __syncthreads();
-    if (valid) {
+  if (slabs_fit(T)) {
+    const int sx = slab_width(T);
+    for (int xs = T.lo[0]; xs < T.lo[0] + T.t[0]; xs += sx) {  // uniform over the workgroup
+      const Tile L = slab(T, xs, sx);
+      if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
+      for (int i = threadIdx.x; i < L.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
+      __syncthreads();
+      if (valid) {
 #if DDM_SCATTER_ROTATE
-      const int start = (int)(threadIdx.x % 27u);
+        const int start = (int)(threadIdx.x % 27u);
 #pragma unroll 1
-      for (int oo = 0; oo < 27; ++oo) {
-        int o = oo + start;
-        o = o >= 27 ? o - 27 : o;
-        int i, j, k;
-        offset_ijk(o, i, j, k);
+        for (int oo = 0; oo < 27; ++oo) {
+          int o = oo + start;
+          o = o >= 27 ? o - 27 : o;
+          int i, j, k;
+          offset_ijk(o, i, j, k);
+          if (!L.has_x(q.st.base[0] + i)) continue;
 #else
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i) {
+          if (!L.has_x(q.st.base[0] + i)) continue;
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+          for (int j = 0; j < 3; ++j)
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < 3; ++k) {
 #endif
-        float cm;
-        f3 cmv;
-        p2g_contrib(q, S.dx, i, j, k, cm, cmv);
-        int li = T.local(q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
-        atomicAdd(&tm[li], cm);
-        atomicAdd(&tx[li], cmv.x);
-        atomicAdd(&ty[li], cmv.y);
-        atomicAdd(&tz[li], cmv.z);
+          float cm;
+          f3 cmv;
+          p2g_contrib(q, S.dx, i, j, k, cm, cmv);
+          int li = L.local(q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
+          atomicAdd(&tm[li], cm);
+          atomicAdd(&tx[li], cmv.x);
+          atomicAdd(&ty[li], cmv.y);
+          atomicAdd(&tz[li], cmv.z);
+#if !DDM_SCATTER_ROTATE
+            }
+#endif
+        }
       }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < T.n; i += kWG) {
-      float a = tm[i], b = tx[i], c = ty[i], d = tz[i];
-      if (a != 0.f || b != 0.f || c != 0.f || d != 0.f) {
-        int x = T.lo[0] + i / (T.t[1] * T.t[2]), y = T.lo[1] + (i / T.t[2]) % T.t[1], z = T.lo[2] + i % T.t[2];
-        float *g = gmmv + 4 * (size_t)node_index(S, x, y, z);
-        atomicAdd(g + 0, a);
-        atomicAdd(g + 1, b);
-        atomicAdd(g + 2, c);
-        atomicAdd(g + 3, d);
+      __syncthreads();
+      for (int i = threadIdx.x; i < L.n; i += kWG) {
+        float a = tm[i], b = tx[i], c = ty[i], d = tz[i];
+        if (a != 0.f || b != 0.f || c != 0.f || d != 0.f) {
+          int x = L.lo[0] + i / (L.t[1] * L.t[2]), y = L.lo[1] + (i / L.t[2]) % L.t[1], z = L.lo[2] + i % L.t[2];
+          float *g = gmmv + 4 * (size_t)node_index(S, x, y, z);
+          atomicAdd(g + 0, a);
+          atomicAdd(g + 1, b);
+          atomicAdd(g + 2, c);
+          atomicAdd(g + 3, d);
+        }
       }
     }
     flag_tile_blocks(S, T, B.block_flags);
@@ -443,8 +467,9 @@ __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__res
 
 struct TileScatter3 {
   float *tx, *ty, *tz;
-  const Tile *T;
+  const Tile *T;  // one x-slab of the chunk box; nodes outside it are skipped
   __device__ void operator()(int x, int y, int z, f3 g) const {
+    if (!T->has_x(x)) return;
     int li = T->local(x, y, z);
     atomicAdd(&tx[li], g.x);
     atomicAdd(&ty[li], g.y);
@@ -485,26 +510,31 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   }
   Tile T = chunk_tile(valid, st.base, s_box);
   BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
-  if (T.n <= kTileMax) {
-    for (int i = threadIdx.x; i < T.n; i += kWG) tx[i] = ty[i] = tz[i] = 0.f;
-    __syncthreads();
-    if (valid) {
-      TileScatter3 sc{tx, ty, tz, &T};
-      f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
-                                ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc, S.inv_dx,
-                                S.dx, S.dt, S.ground_height, dim,
-                                DDM_SCATTER_ROTATE ? (int)(threadIdx.x % 27u) : -1);
-      add3(cur_grad.x + 3 * (size_t)p, gxp);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < T.n; i += kWG) {
-      float a = tx[i], b = ty[i], c = tz[i];
-      if (a != 0.f || b != 0.f || c != 0.f) {
-        int xx = T.lo[0] + i / (T.t[1] * T.t[2]), yy = T.lo[1] + (i / T.t[2]) % T.t[1], zz = T.lo[2] + i % T.t[2];
-        float *g = B.grid_dv + 4 * (size_t)node_index(S, xx, yy, zz);
-        atomicAdd(g + 0, a);
-        atomicAdd(g + 1, b);
-        atomicAdd(g + 2, c);
+  if (slabs_fit(T)) {
+    const int sx = slab_width(T);
+    for (int xs = T.lo[0]; xs < T.lo[0] + T.t[0]; xs += sx) {  // uniform over the workgroup
+      const Tile L = slab(T, xs, sx);
+      if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
+      for (int i = threadIdx.x; i < L.n; i += kWG) tx[i] = ty[i] = tz[i] = 0.f;
+      __syncthreads();
+      if (valid) {
+        TileScatter3 sc{tx, ty, tz, &L};
+        f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
+                                  ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc,
+                                  S.inv_dx, S.dx, S.dt, S.ground_height, dim,
+                                  DDM_SCATTER_ROTATE ? (int)(threadIdx.x % 27u) : -1);
+        if (xs == T.lo[0]) add3(cur_grad.x + 3 * (size_t)p, gxp);  // every pass computes the same gxp
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < L.n; i += kWG) {
+        float a = tx[i], b = ty[i], c = tz[i];
+        if (a != 0.f || b != 0.f || c != 0.f) {
+          int xx = L.lo[0] + i / (L.t[1] * L.t[2]), yy = L.lo[1] + (i / L.t[2]) % L.t[1], zz = L.lo[2] + i % L.t[2];
+          float *g = B.grid_dv + 4 * (size_t)node_index(S, xx, yy, zz);
+          atomicAdd(g + 0, a);
+          atomicAdd(g + 1, b);
+          atomicAdd(g + 2, c);
+        }
       }
     }
   } else if (valid) {

@@ -375,8 +375,10 @@ class RolloutServer:
             self._launch(bwd, "B")
         for a, _ in fwd + bwd:
             a.pc += 1
-        self.ticks += 1
+        if fwd or bwd:
+            self.ticks += 1  # ticks that launched work
         self._collect()
+        return bool(fwd or bwd)
 
     def _collect(self, block=False):
         still = []
@@ -403,7 +405,8 @@ class RolloutServer:
         while self.waiting or self.active:
             if not self.active and self.waiting and self.waiting[0].arrival > self._now():
                 time.sleep(min(1e-3, self.waiting[0].arrival - self._now()))
-            self.tick()
+            if not self.tick() and self.active and all(a.done_event is not None for a in self.active):
+                self._collect(block=True)  # nothing left to launch: wait for the stream
             if self._now() > timeout:
                 raise TimeoutError("rollout server did not drain in time")
         torch.cuda.synchronize()

@@ -170,6 +170,50 @@ def test_fused_backward_one_substep(plain, mode):
             assert e <= 1e-3, (name, e)
 
 
+@pytest.mark.parametrize("layout", ["slabs", "global"])
+def test_unsorted_particles_large_chunk_boxes(layout):
+    """Particles in random order, so a 256-particle chunk's stencil box exceeds the
+    1024-node LDS tile. "slabs": a 0.2-wide box (about 16^3 nodes per chunk box, 256
+    nodes per x-plane) is scattered in several x-slab passes; "global": the box spans
+    more than 32 nodes in y and z, one x-plane exceeds the tile and the chunk scatters
+    with global atomics. One forward and one adjoint substep against the oracle."""
+    grid = 64
+    lo, hi = ([0.4, 0.05, 0.4], [0.6, 0.25, 0.6]) if layout == "slabs" else ([0.3, 0.05, 0.2], [0.7, 0.62, 0.8])
+    rng = np.random.default_rng(11)
+    x0 = rng.uniform(lo, hi, size=(8000, 3)).astype(np.float32)
+    pos = np.float32([[0.45, hi[1] + 0.02, 0.5], [0.55, hi[1] + 0.02, 0.5]])
+    sc = W.make_scene(x0, grid, 1.0 / grid, 5e-5, W.capsule_prims(2), [[0.03, 0.1, 0, 0]] * 2, pos,
+                      [[1, 0, 0, 0]] * 2)
+    st = initial_state(sc, rng, v_scale=0.2, f_scale=0.02, c_scale=0.5)
+    P1, R1 = sc.body_pos + np.float32([0, -2e-4, 0]), sc.body_rot.copy()
+    osim = O.OracleSim(oracle_scene(sc), np.float32)
+    ref = osim.substep(st, P1, R1)
+    eng = _engine(sc)
+    cur, nxt = eng.new_state(), eng.new_state()
+    _to_buf(st, cur)
+    nxt.body_pos.copy_(torch.from_numpy(P1))
+    nxt.body_rot.copy_(torch.from_numpy(R1))
+    eng.forward(cur, nxt)
+    torch.cuda.synchronize()
+    for name, g, r in zip("xvFC", _np(nxt), (ref.x, ref.v, ref.F, ref.C)):
+        e = rel_err(g, r)
+        print(f"{layout} substep {name}: rel err {e:.3e}")
+        assert e <= 1e-5, (name, e)
+    g_next = _random_grads(sc.n, sc.nb, seed=7)
+    bounds = _fp32_noise_bound(sc, st, ref, g_next, 0)
+    (g32, _), (g64, _) = bounds[np.float32], bounds[np.float64]
+    gcur, gnxt = eng.new_grad(), eng.new_grad()
+    _to_buf(ref, nxt)
+    _to_buf(O.State(g_next.x, g_next.v, g_next.F, g_next.C, g_next.body_pos, g_next.body_rot), gnxt)
+    eng.backward(cur, nxt, gcur, gnxt)
+    torch.cuda.synchronize()
+    for name, g, a, b in zip("xvFC", _np(gcur), (g32.x, g32.v, g32.F, g32.C), (g64.x, g64.v, g64.F, g64.C)):
+        e = rel_err(g, a)
+        noise = rel_err(a, b)
+        print(f"{layout} adjoint {name}: rel err {e:.3e} (fp32 noise {noise:.3e})")
+        assert e <= max(1e-3, 2 * noise), (name, e)
+
+
 def _simulator(sc):
     from dexdeform_amd.mpm.simulator import MPMSimulator
     s = sc.spec
