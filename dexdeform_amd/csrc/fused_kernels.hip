@@ -10,10 +10,15 @@
 //    visit only flagged blocks. Between substeps grid_mmv, grid_dv and block_flags are
 //    all zero: the last grid kernel of a substep clears what it consumed, so no dense
 //    memset is ever needed.
-//  * P2G and the G2P adjoint scatter into an LDS tile covering the bounding box of the
-//    chunk's stencils and flush only the non-zero tile nodes with global atomics. A box
+//  * P2G and the G2P adjoint stage each particle's stencil data in LDS, group the chunk
+//    into runs of consecutive particles with the same stencil base (cells, for
+//    cell-sorted particles) and scatter one (run, stencil node) task per thread: the
+//    task sums its run in registers and adds one value per component to an LDS tile
+//    covering the chunk's stencil box. LDS float atomics are the throughput limit of
+//    this scatter, so a run of n particles costs one atomic instead of n.
+//  * The tile is flushed to the grid with global atomics for its non-zero nodes. A box
 //    larger than the tile is processed in x-slabs, one tile pass per slab; only a box
-//    whose single x-plane exceeds the tile scatters with global atomics directly.
+//    whose single x-plane exceeds the tile sends the tasks' sums to global atomics.
 //  * The contact adjoint reduces body-pose gradients across the wavefront before one
 //    atomic per body per wave.
 #include <limits.h>
@@ -25,7 +30,12 @@ using namespace ddm;
 namespace {
 
 constexpr int kWG = 256;
-constexpr int kTileMax = 1024;   // LDS tile capacity in nodes (16 KB for 4 floats)
+// LDS tile capacity in nodes. P2G keeps 4 floats per node and the G2P adjoint 3; with
+// the staged particle records both stay under 40 KB of LDS per workgroup, so four
+// workgroups (one wave per SIMD each) fit a CU.
+constexpr int kTileP2G = 704;
+constexpr int kTileG2P = 960;
+constexpr int kHash = 512;       // open-addressing table of the chunk's stencil bases
 constexpr int kTapeMax = 8;      // contact tape entries kept per node in the adjoint
 
 __device__ __forceinline__ int node_index(const ddm_scene &S, int x, int y, int z) {
@@ -62,9 +72,11 @@ struct Tile {
   __device__ bool has_x(int x) const { return x >= lo[0] && x < lo[0] + t[0]; }
 };
 // nodes per x-plane of the chunk box must fit the tile for the slab passes
-__device__ __forceinline__ bool slabs_fit(const Tile &T) { return T.t[1] * T.t[2] <= kTileMax; }
+template <int CAP>
+__device__ __forceinline__ bool slabs_fit(const Tile &T) { return T.t[1] * T.t[2] <= CAP; }
 // x-planes per slab pass
-__device__ __forceinline__ int slab_width(const Tile &T) { return min(T.t[0], kTileMax / (T.t[1] * T.t[2])); }
+template <int CAP>
+__device__ __forceinline__ int slab_width(const Tile &T) { return min(T.t[0], CAP / (T.t[1] * T.t[2])); }
 // the slab of the chunk box starting at x-plane xs
 __device__ __forceinline__ Tile slab(const Tile &T, int xs, int sx) {
   Tile L = T;
@@ -74,28 +86,156 @@ __device__ __forceinline__ Tile slab(const Tile &T, int xs, int sx) {
   return L;
 }
 
-// Scatter strategy (compile-time):
-//  DDM_SCATTER_ROTATE = 1: each lane visits the 27 stencil nodes starting at a
-//    lane-dependent offset, so lanes whose particles share a cell hit distinct LDS
-//    addresses in each instruction; 0: all lanes visit in the reference's order with the
-//    loops fully unrolled (cheaper address and weight arithmetic).
-//  DDM_INTERLEAVE = 1: lane l of wave w handles particle 4*l + w of its 256-particle
-//    chunk, so particles adjacent in the (spatially sorted) order land in different
-//    waves; 0: lane = particle.
-#ifndef DDM_SCATTER_ROTATE
-#define DDM_SCATTER_ROTATE 1
-#endif
-#ifndef DDM_INTERLEAVE
-#define DDM_INTERLEAVE 0
-#endif
-
-__device__ __forceinline__ int chunk_slot(int t) {
-#if DDM_INTERLEAVE
-  return ((t & 63) << 2) | (t >> 6);
-#else
-  return t;
-#endif
+// ---------------------------------------------------------------- stencil groups
+// Particles of a chunk that share a stencil base form a group. A scatter task is one
+// (group, stencil node) pair: it sums the group's contributions to that node in
+// registers and adds the sum to the tile with one atomic per component, instead of
+// one atomic per particle.
+//
+// Per-particle stencil data staged in LDS, in group order. f1..f3 hold a vector and a
+// row-major 3x3 matrix whose meaning is the kernel's (P2G: m v and the affine momentum
+// matrix; G2P adjoint: node-velocity adjoint seed and C adjoint).
+struct StencilRecs {
+  float4 f0[kWG];    // fx (3), scalar
+  float4 f1[kWG];    // vector (3), matrix[0]
+  float4 f2[kWG];    // matrix[1..4]
+  float4 f3[kWG];    // matrix[5..8]
+  float w[kWG * 9];  // B-spline weights w[axis][offset], 9 consecutive per particle
+};
+__device__ __forceinline__ void stage_record(StencilRecs &rc, int s, const Stencil &st, float scalar, f3 vec,
+                                             const m33 &M) {
+  rc.f0[s] = make_float4(st.fx[0], st.fx[1], st.fx[2], scalar);
+  rc.f1[s] = make_float4(vec.x, vec.y, vec.z, M.a[0]);
+  rc.f2[s] = make_float4(M.a[1], M.a[2], M.a[3], M.a[4]);
+  rc.f3[s] = make_float4(M.a[5], M.a[6], M.a[7], M.a[8]);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) rc.w[9 * s + e] = st.w[e / 3][e % 3];
 }
+__device__ __forceinline__ f3 rec_vec(const StencilRecs &rc, int s) {
+  float4 b = rc.f1[s];
+  return f3{b.x, b.y, b.z};
+}
+__device__ __forceinline__ m33 rec_mat(const StencilRecs &rc, int s) {
+  float4 b = rc.f1[s], c = rc.f2[s], d = rc.f3[s];
+  return m33{{b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w}};
+}
+// stencil base packed into one non-negative int (grid dimensions are at most 1024)
+__device__ __forceinline__ int pack_base(const int b[3]) { return (b[0] << 20) | (b[1] << 10) | b[2]; }
+__device__ __forceinline__ void unpack_base(int key, int b[3]) {
+  b[0] = key >> 20;
+  b[1] = (key >> 10) & 1023;
+  b[2] = key & 1023;
+}
+
+// LDS used only while grouping; it shares storage with the scatter tile.
+struct GroupScratch {
+  int hkey[kHash];  // base key per table entry, -1 when empty
+  int hgrp[kHash];  // group index of the entry
+  int cnt[kWG];     // group sizes
+};
+// Group layout shared with the scatter tasks.
+struct Groups {
+  int off[kWG + 1];  // group g holds record slots off[g] .. off[g+1]-1
+  int key[kWG];      // packed stencil base of group g
+  int wcnt[kWG / 64];
+};
+
+// Groups the chunk's valid slots (a prefix of the chunk) by stencil base. Returns the
+// number of groups G (uniform) and, for a valid slot, the record slot pos where its
+// data goes: groups are contiguous and in the order their first member was inserted.
+// Ends with a barrier, after which the scratch is free.
+__device__ int build_groups(bool valid, int key, int n_valid, GroupScratch &sc, Groups &gr, int &pos) {
+  const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int i = t; i < kHash; i += kWG) sc.hkey[i] = -1;
+  sc.cnt[t] = 0;
+  __syncthreads();
+  int h = 0;
+  bool leader = false;
+  if (valid) {
+    // the low 3 bits of each coordinate: collision-free for bases spanning <= 8 cells per axis
+    h = ((key >> 14) & 0x1c0) | ((key >> 7) & 0x38) | (key & 7);
+    for (int probe = 0; probe < kHash; ++probe) {  // at most kWG keys: a free entry exists
+      const int old = atomicCAS(&sc.hkey[h], -1, key);
+      if (old == -1) {
+        leader = true;
+        break;
+      }
+      if (old == key) break;
+      h = (h + 1) & (kHash - 1);
+    }
+  }
+  unsigned long long m = __ballot(leader);
+  if (lane == 0) gr.wcnt[wave] = __popcll(m);
+  __syncthreads();
+  int G = 0, off = 0;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) {
+    const int c = gr.wcnt[w];
+    off += w < wave ? c : 0;
+    G += c;
+  }
+  if (leader) {
+    const int g = off + __popcll(m & ((1ull << lane) - 1ull));
+    sc.hgrp[h] = g;
+    gr.key[g] = key;
+  }
+  __syncthreads();
+  int g = 0, rank = 0;
+  if (valid) {
+    g = sc.hgrp[h];
+    rank = atomicAdd(&sc.cnt[g], 1);
+  }
+  __syncthreads();
+  // exclusive scan of the group sizes
+  const int c = t < G ? sc.cnt[t] : 0;
+  int v = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) gr.wcnt[wave] = v;
+  __syncthreads();
+  off = 0;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) off += w < wave ? gr.wcnt[w] : 0;
+  if (t < G) gr.off[t] = off + v - c;
+  if (t == 0) gr.off[G] = n_valid;
+  __syncthreads();
+  pos = valid ? gr.off[g] + rank : 0;
+  return G;
+}
+
+// Every (group, stencil node) task whose node lies in slab L (all tasks when L is null):
+// acc = sum over the group's records of contrib(rc, slot, i, j, k, w), then
+// sink(X, Y, Z, acc). Tasks are spread over the workgroup's threads.
+template <int NC, class Contrib, class Sink>
+__device__ void group_tasks(int G, const Groups &gr, const StencilRecs &rc, const Tile *L, Contrib contrib,
+                            Sink sink) {
+  for (int task = (int)threadIdx.x; task < 27 * G; task += kWG) {
+    const int g = task / 27, o = task - 27 * (task / 27);
+    int i, j, k;
+    offset_ijk(o, i, j, k);
+    int b[3];
+    unpack_base(gr.key[g], b);
+    if (L && !L->has_x(b[0] + i)) continue;
+    float acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = 0.f;
+    for (int q = gr.off[g], qe = gr.off[g + 1]; q < qe; ++q) {
+      const float w = rc.w[9 * q + i] * rc.w[9 * q + 3 + j] * rc.w[9 * q + 6 + k];  // weight_dyn's order
+      contrib(rc, q, i, j, k, w, acc);
+    }
+    sink(b[0] + i, b[1] + j, b[2] + k, acc);
+  }
+}
+
+// the scatter tile shares LDS with the grouping scratch, which is dead by then
+template <int NC, int CAP>
+union ScatterLDS {
+  float tile[NC][CAP];
+  GroupScratch grp;
+};
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -169,15 +309,19 @@ __device__ void flag_stencil_blocks(const ddm_scene &S, const int base[3], uint3
 template <bool BWD>
 __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                              const ddm_state *__restrict__ next_tab) {
-  __shared__ float tm[kTileMax], tx[kTileMax], ty[kTileMax], tz[kTileMax];
+  __shared__ ScatterLDS<4, kTileP2G> sh;
+  __shared__ StencilRecs rc;
+  __shared__ Groups gr;
   __shared__ int s_box[6];
+  float *const tm = sh.tile[0], *const tx = sh.tile[1], *const ty = sh.tile[2], *const tz = sh.tile[3];
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x];
-  const int p = ch.y + chunk_slot((int)threadIdx.x);
+  const int p = ch.y + (int)threadIdx.x;
   const bool valid = p < ch.z;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   P2GParticle q;
+  int key = 0;
   if (valid) {
     m33 F = ld9(cur.F + 9 * (size_t)p), C = ld9(cur.C + 9 * (size_t)p);
     m33 Ft = advect_F(F, C, S.dt);
@@ -195,50 +339,40 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
     } else {
       st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     }
+    key = pack_base(q.st.base);
   } else {
     q.st.base[0] = q.st.base[1] = q.st.base[2] = 0;
   }
-  Tile T = chunk_tile(valid, q.st.base, s_box);
+  const Tile T = chunk_tile(valid, q.st.base, s_box);
+  int pos;
+  const int G = build_groups(valid, key, min(kWG, ch.z - ch.y), sh.grp, gr, pos);
+  if (valid) stage_record(rc, pos, q.st, q.m, q.mv, q.affine);
+  const float dx = S.dx;
+  auto contrib = [dx](const StencilRecs &r, int s, int i, int j, int k, float w, float *acc) {
+    float cm;
+    f3 cmv;
+    float4 a = r.f0[s];
+    p2g_node_contrib(w, f3{a.x, a.y, a.z}, a.w, rec_vec(r, s), rec_mat(r, s), dx, i, j, k, cm, cmv);
+    acc[0] += cm;
+    acc[1] += cmv.x;
+    acc[2] += cmv.y;
+    acc[3] += cmv.z;
+  };
   float *gmmv = B.grid_mmv;
-  if (slabs_fit(T)) {
-    const int sx = slab_width(T);
+  if (slabs_fit<kTileP2G>(T)) {
+    const int sx = slab_width<kTileP2G>(T);
     for (int xs = T.lo[0]; xs < T.lo[0] + T.t[0]; xs += sx) {  // uniform over the workgroup
       const Tile L = slab(T, xs, sx);
       if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
       for (int i = threadIdx.x; i < L.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
-      __syncthreads();
-      if (valid) {
-#if DDM_SCATTER_ROTATE
-        const int start = (int)(threadIdx.x % 27u);
-#pragma unroll 1
-        for (int oo = 0; oo < 27; ++oo) {
-          int o = oo + start;
-          o = o >= 27 ? o - 27 : o;
-          int i, j, k;
-          offset_ijk(o, i, j, k);
-          if (!L.has_x(q.st.base[0] + i)) continue;
-#else
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          if (!L.has_x(q.st.base[0] + i)) continue;
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-#endif
-          float cm;
-          f3 cmv;
-          p2g_contrib(q, S.dx, i, j, k, cm, cmv);
-          int li = L.local(q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
-          atomicAdd(&tm[li], cm);
-          atomicAdd(&tx[li], cmv.x);
-          atomicAdd(&ty[li], cmv.y);
-          atomicAdd(&tz[li], cmv.z);
-#if !DDM_SCATTER_ROTATE
-            }
-#endif
-        }
-      }
+      __syncthreads();  // also publishes the staged records
+      group_tasks<4>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
+        const int li = L.local(X, Y, Z);
+        atomicAdd(&tm[li], a[0]);
+        atomicAdd(&tx[li], a[1]);
+        atomicAdd(&ty[li], a[2]);
+        atomicAdd(&tz[li], a[3]);
+      });
       __syncthreads();
       for (int i = threadIdx.x; i < L.n; i += kWG) {
         float a = tm[i], b = tx[i], c = ty[i], d = tz[i];
@@ -253,23 +387,20 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
       }
     }
     flag_tile_blocks(S, T, B.block_flags);
-  } else if (valid) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          float cm;
-          f3 cmv;
-          p2g_contrib(q, S.dx, i, j, k, cm, cmv);
-          float *g = gmmv + 4 * (size_t)node_index(S, q.st.base[0] + i, q.st.base[1] + j, q.st.base[2] + k);
-          atomicAdd(g + 0, cm);
-          atomicAdd(g + 1, cmv.x);
-          atomicAdd(g + 2, cmv.y);
-          atomicAdd(g + 3, cmv.z);
-        }
-    flag_stencil_blocks(S, q.st.base, B.block_flags);
+  } else {
+    __syncthreads();  // publishes the staged records
+    group_tasks<4>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
+      float *g = gmmv + 4 * (size_t)node_index(S, X, Y, Z);
+      atomicAdd(g + 0, a[0]);
+      atomicAdd(g + 1, a[1]);
+      atomicAdd(g + 2, a[2]);
+      atomicAdd(g + 3, a[3]);
+    });
+    for (int g = threadIdx.x; g < G; g += kWG) {
+      int b[3];
+      unpack_base(gr.key[g], b);
+      flag_stencil_blocks(S, b, B.block_flags);
+    }
   }
 }
 
@@ -453,7 +584,7 @@ __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__res
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
-  const int p = ch.y + chunk_slot((int)threadIdx.x);
+  const int p = ch.y + (int)threadIdx.x;
   if (p >= ch.z) return;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
@@ -465,66 +596,68 @@ __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__res
   st9(next.C + 9 * (size_t)p, Co);
 }
 
-struct TileScatter3 {
-  float *tx, *ty, *tz;
-  const Tile *T;  // one x-slab of the chunk box; nodes outside it are skipped
-  __device__ void operator()(int x, int y, int z, f3 g) const {
-    if (!T->has_x(x)) return;
-    int li = T->local(x, y, z);
-    atomicAdd(&tx[li], g.x);
-    atomicAdd(&ty[li], g.y);
-    atomicAdd(&tz[li], g.z);
-  }
-};
-struct GlobalScatter3 {
-  float *g;
-  const ddm_scene *S;
-  __device__ void operator()(int x, int y, int z, f3 v) const {
-    float *p = g + 4 * (size_t)node_index(*S, x, y, z);
-    atomicAdd(p + 0, v.x);
-    atomicAdd(p + 1, v.y);
-    atomicAdd(p + 2, v.z);
-  }
+struct NullScatter {
+  __device__ void operator()(int, int, int, f3) const {}
 };
 
-// adjoint of G2P: scatters the node-velocity adjoint, adds to the x adjoint
+// adjoint of G2P: adds to the x adjoint and scatters the node-velocity adjoint
 __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ next_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
-  __shared__ float tx[kTileMax], ty[kTileMax], tz[kTileMax];
+  __shared__ ScatterLDS<3, kTileG2P> sh;
+  __shared__ StencilRecs rc;
+  __shared__ Groups gr;
   __shared__ int s_box[6];
+  float *const tx = sh.tile[0], *const ty = sh.tile[1], *const tz = sh.tile[2];
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
   const ddm_state cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
-  const int p = ch.y + chunk_slot((int)threadIdx.x);
+  const int p = ch.y + (int)threadIdx.x;
   const bool valid = p < ch.z;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
-  f3 x{0.f, 0.f, 0.f};
   Stencil st;
-  st.base[0] = st.base[1] = st.base[2] = 0;
+  f3 g_nv{0.f, 0.f, 0.f};
+  m33 gC;
+  int key = 0;
   if (valid) {
-    x = ld3(cur.x + 3 * (size_t)p);
-    stencil(x, S.inv_dx, dim, st);
+    BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
+    NullScatter none;
+    gC = ld9(next_grad.C + 9 * (size_t)p);
+    f3 gxp = g2p_particle_adj(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
+                              ld3(next_grad.x + 3 * (size_t)p), ld3(next_grad.v + 3 * (size_t)p), gC, gv, none,
+                              S.inv_dx, S.dx, S.dt, S.ground_height, dim, &g_nv, &st);
+    add3(cur_grad.x + 3 * (size_t)p, gxp);
+    key = pack_base(st.base);
+  } else {
+    st.base[0] = st.base[1] = st.base[2] = 0;
   }
-  Tile T = chunk_tile(valid, st.base, s_box);
-  BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
-  if (slabs_fit(T)) {
-    const int sx = slab_width(T);
+  const Tile T = chunk_tile(valid, st.base, s_box);
+  int pos;
+  const int G = build_groups(valid, key, min(kWG, ch.z - ch.y), sh.grp, gr, pos);
+  if (valid) stage_record(rc, pos, st, 0.f, g_nv, gC);
+  const float inv_dx = S.inv_dx;
+  auto contrib = [inv_dx](const StencilRecs &r, int s, int i, int j, int k, float w, float *acc) {
+    float4 a = r.f0[s];
+    f3 g = g2p_adj_contrib(w, f3{a.x, a.y, a.z}, rec_vec(r, s), rec_mat(r, s), inv_dx, i, j, k);
+    acc[0] += g.x;
+    acc[1] += g.y;
+    acc[2] += g.z;
+  };
+  if (slabs_fit<kTileG2P>(T)) {
+    const int sx = slab_width<kTileG2P>(T);
     for (int xs = T.lo[0]; xs < T.lo[0] + T.t[0]; xs += sx) {  // uniform over the workgroup
       const Tile L = slab(T, xs, sx);
       if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
       for (int i = threadIdx.x; i < L.n; i += kWG) tx[i] = ty[i] = tz[i] = 0.f;
-      __syncthreads();
-      if (valid) {
-        TileScatter3 sc{tx, ty, tz, &L};
-        f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
-                                  ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc,
-                                  S.inv_dx, S.dx, S.dt, S.ground_height, dim,
-                                  DDM_SCATTER_ROTATE ? (int)(threadIdx.x % 27u) : -1);
-        if (xs == T.lo[0]) add3(cur_grad.x + 3 * (size_t)p, gxp);  // every pass computes the same gxp
-      }
+      __syncthreads();  // also publishes the staged records
+      group_tasks<3>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
+        const int li = L.local(X, Y, Z);
+        atomicAdd(&tx[li], a[0]);
+        atomicAdd(&ty[li], a[1]);
+        atomicAdd(&tz[li], a[2]);
+      });
       __syncthreads();
       for (int i = threadIdx.x; i < L.n; i += kWG) {
         float a = tx[i], b = ty[i], c = tz[i];
@@ -537,12 +670,14 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
         }
       }
     }
-  } else if (valid) {
-    GlobalScatter3 sc{B.grid_dv, &S};
-    f3 gxp = g2p_particle_adj(x, ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
-                              ld3(next_grad.v + 3 * (size_t)p), ld9(next_grad.C + 9 * (size_t)p), gv, sc, S.inv_dx,
-                              S.dx, S.dt, S.ground_height, dim);
-    add3(cur_grad.x + 3 * (size_t)p, gxp);
+  } else {
+    __syncthreads();  // publishes the staged records
+    group_tasks<3>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
+      float *g = B.grid_dv + 4 * (size_t)node_index(S, X, Y, Z);
+      atomicAdd(g + 0, a[0]);
+      atomicAdd(g + 1, a[1]);
+      atomicAdd(g + 2, a[2]);
+    });
   }
 }
 
@@ -553,7 +688,7 @@ __global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, const ddm_state *_
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
-  const int p = ch.y + chunk_slot((int)threadIdx.x);
+  const int p = ch.y + (int)threadIdx.x;
   if (p >= ch.z) return;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   const size_t p3 = 3 * (size_t)p, p9 = 9 * (size_t)p;

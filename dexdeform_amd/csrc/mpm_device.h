@@ -554,12 +554,17 @@ DDM_HD float weight_dyn(const Stencil &s, int i, int j, int k) {
          sel3(k, s.w[2][0], s.w[2][1], s.w[2][2]);
 }
 
-// contribution of one particle to stencil node (i,j,k): (m w, w (m v + affine dpos))
+// contribution of one particle with stencil weight w to stencil node (i,j,k):
+// (m w, w (m v + affine dpos))
+DDM_HD void p2g_node_contrib(float w, f3 fx, float m, f3 mv, const m33 &affine, float dx, int i, int j, int k,
+                             float &cm, f3 &cmv) {
+  f3 dpos = f3{((float)i - fx.x) * dx, ((float)j - fx.y) * dx, ((float)k - fx.z) * dx};
+  cm = m * w;
+  cmv = (mv + mvec(affine, dpos)) * w;
+}
 DDM_HD void p2g_contrib(const P2GParticle &p, float dx, int i, int j, int k, float &cm, f3 &cmv) {
-  float w = weight_dyn(p.st, i, j, k);
-  f3 dpos = f3{((float)i - p.st.fx[0]) * dx, ((float)j - p.st.fx[1]) * dx, ((float)k - p.st.fx[2]) * dx};
-  cm = p.m * w;
-  cmv = (p.mv + mvec(p.affine, dpos)) * w;
+  p2g_node_contrib(weight_dyn(p.st, i, j, k), f3{p.st.fx[0], p.st.fx[1], p.st.fx[2]}, p.m, p.mv, p.affine, dx, i, j,
+                   k, cm, cmv);
 }
 
 // ------------------------------------------------------------------ grid node update
