@@ -6,8 +6,10 @@ the single-scene path.
 
 Workload (default, "batched"): 64 scenes, particle counts U[5k, 60k], 64^3 grid each,
 yield stress U[20, 100], 2-22 capsules per scene with constant random velocities,
-100 substeps forward then 100 adjoint substeps, all scenes in one launch per stage.
-A step = one such rollout (forward + backward) of all 64 scenes.
+100 substeps forward then 100 adjoint substeps, all scenes in one launch per stage,
+the whole rollout captured once into a HIP graph and replayed. A step = one such
+rollout (forward + backward) of all 64 scenes. Also reported: the same rollout with
+eager launches, and the scenes run one at a time (plain; eager and graph-replayed).
 
 Prints one JSON line (rank 0). --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -75,6 +77,35 @@ def build_rollout(scenes, poses, T, device, checkpoint_every=None):
     gx = torch.tensor(rng.normal(size=(eng.n_particles_total, 3)).astype(np.float32) / eng.n_particles_total,
                       device=device)
     return eng, ro, gx
+
+
+def serve_benchmark(dev, rates, n_requests, seed):
+    """Served rollouts: for each Poisson arrival rate, a stream of uneven requests
+    (particles U[5k,100k], grid 48^3/64^3/96^3, horizon U[20,200] with checkpoints every
+    10 substeps, forward + backward each) through the continuous-batching server.
+    Latency = completion - arrival; throughput = particle-substeps (forward + adjoint
+    counted once) per second of wall time from the first arrival to the last completion."""
+    import torch
+    from dexdeform_amd.mpm import workloads as W
+    from dexdeform_amd.mpm.serve import Request, RolloutServer, latency_stats
+    out = []
+    for k, rate in enumerate(rates):
+        stream = W.served_stream(rate, n_requests, seed=seed + 1000 * k)
+        reqs = [Request(i, d["scene"].spec, d["scene"].x0, d["pose_pos"], d["pose_rot"], d["target"],
+                        arrival=d["arrival"], checkpoint_every=d["checkpoint_every"]) for i, d in enumerate(stream)]
+        srv = RolloutServer(device=dev)
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        st = latency_stats(res)
+        ps = sum(r.n * r.T for r in reqs)
+        out.append({"rate_req_s": rate, "requests": n_requests, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                    "throughput": ps / wall, "wall_s": wall, "ticks": srv.ticks})
+        log(f"serve rate {rate}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
+            f"throughput {ps / wall:.3e} particle-substeps/s")
+        del srv, res
+        torch.cuda.empty_cache()
+    return out
 
 
 def run_step(ro, gx):
@@ -183,6 +214,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="10,50,100,200", help="request arrival rates (req/s) to sweep")
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests per rate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -205,7 +239,7 @@ def main():
     n_total = sum(s.n for s in scenes)
     log(f"batched: {len(scenes)} scenes, {n_total} particles, {sum(s.nb for s in scenes)} bodies, T={T}")
 
-    # ---------------- batched
+    # ---------------- batched: eager launches, then the same rollout replayed from a HIP graph
     eng, ro, gx = build_rollout(scenes, poses, T, dev)
     for _ in range(args.warmup):
         run_step(ro, gx)
@@ -214,7 +248,18 @@ def main():
     for i in range(args.steps):
         run_step(ro, gx)
         torch.cuda.synchronize()
-        log(f"batched step {i + 1}/{args.steps} done")
+        log(f"batched eager step {i + 1}/{args.steps} done")
+    t1 = time.perf_counter()
+    eager_value = n_total * T / ((t1 - t0) / args.steps)
+    ro.capture(gx)
+    for _ in range(args.warmup):
+        ro.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ro.replay()
+        torch.cuda.synchronize()
+        log(f"batched graph step {i + 1}/{args.steps} done")
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = n_total * T / ((t1 - t0) / args.steps)
@@ -226,10 +271,11 @@ def main():
     sum_ms = sum(kt.values())
     alg = 480.0 * n_total / (sum_ms * 1e-3) / 1e9
 
-    # ---------------- plain: the same scenes one at a time through the single-scene path
-    plain_value = None
+    # ---------------- plain: the same scenes one at a time through the single-scene path,
+    # eager and replayed from a per-scene HIP graph
+    plain_value = plain_eager_value = None
     if not args.no_plain:
-        tot_t, tot_ps = 0.0, 0
+        tot_e, tot_g, tot_ps = 0.0, 0.0, 0
         for i, (sc, po) in enumerate(zip(scenes, poses)):
             e1, r1, g1 = build_rollout([sc], [po], T, dev)
             run_step(r1, g1)
@@ -237,12 +283,24 @@ def main():
             a = time.perf_counter()
             run_step(r1, g1)
             torch.cuda.synchronize()
-            tot_t += time.perf_counter() - a
+            tot_e += time.perf_counter() - a
+            r1.capture(g1)
+            r1.replay()
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            r1.replay()
+            torch.cuda.synchronize()
+            tot_g += time.perf_counter() - a
             tot_ps += sc.n * T
             del e1, r1, g1
             if (i + 1) % 16 == 0:
                 log(f"plain {i + 1}/{len(scenes)} scenes done")
-        plain_value = tot_ps / tot_t
+        plain_value = tot_ps / tot_g
+        plain_eager_value = tot_ps / tot_e
+
+    serve = None
+    if not args.no_serve:
+        serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_requests, args.seed)
 
     cpu = None
     if not args.no_cpu:
@@ -265,12 +323,15 @@ def main():
         "config": {"workload": f"batched uneven scenes: {len(scenes)} scenes, particles U[5k,60k] "
                                f"(total {n_total}), 64^3 grid each, 2-22 capsules, {T} substeps fwd+bwd",
                    "global_batch": len(scenes), "substeps": T, "particles": n_total, "parallelism": "none"},
+        "eager_value": eager_value,
         "plain_value": plain_value,
+        "plain_eager_value": plain_eager_value,
         "batched_over_plain": (value / plain_value) if plain_value else None,
         "kernel_ms_per_launch": kt,
         "algorithmic_fwd_bwd_GBps": alg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "serve": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -6,10 +6,11 @@
 //    maps each workgroup to <= 256 particles of one scene (local indices).
 //  * Each scene's grid is stored in 4x4x4 node blocks (64 nodes = one wavefront),
 //    blocks of all scenes concatenated; every grid array is float4 per node.
-//  * Block-sparse: P2G marks the blocks it touches in block_flags; the grid kernels
-//    visit only flagged blocks. Between substeps grid_mmv, grid_dv and block_flags are
-//    all zero: the last grid kernel of a substep clears what it consumed, so no dense
-//    memset is ever needed.
+//  * Block-sparse: P2G marks the blocks it touches in block_flags and appends each newly
+//    marked block, with its scene, to block_list; the grid kernels visit exactly the
+//    listed blocks. Between substeps grid_mmv, grid_dv, block_flags and block_count are
+//    all zero: the last grid kernel of a substep clears what it consumed and the kernel
+//    after it resets the count, so no dense memset is ever needed.
 //  * P2G and the G2P adjoint stage each particle's stencil data in LDS, group the chunk
 //    into runs of consecutive particles with the same stencil base (cells, for
 //    cell-sorted particles) and scatter one (run, stencil node) task per thread: the
@@ -281,8 +282,15 @@ __device__ Tile chunk_tile(bool valid, const int base[3], int *s_box) {
   return T;
 }
 
+// mark a block of scene si; the first marker of a substep appends it to the block list
+__device__ __forceinline__ void mark_block(const ddm_batch &B, int blk, int si) {
+  if (atomicExch(&B.block_flags[blk], 1u) == 0u) {
+    const int e = atomicAdd(B.block_count, 1);
+    reinterpret_cast<int2 *>(B.block_list)[e] = make_int2(blk, si);
+  }
+}
 // mark every node block that intersects the tile
-__device__ void flag_tile_blocks(const ddm_scene &S, const Tile &T, uint32_t *flags) {
+__device__ void flag_tile_blocks(const ddm_batch &B, const ddm_scene &S, int si, const Tile &T) {
   int b0[3], nb[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -292,15 +300,15 @@ __device__ void flag_tile_blocks(const ddm_scene &S, const Tile &T, uint32_t *fl
   int tot = nb[0] * nb[1] * nb[2];
   for (int i = threadIdx.x; i < tot; i += kWG) {
     int bx = i / (nb[1] * nb[2]), by = (i / nb[2]) % nb[1], bz = i % nb[2];
-    flags[block_of(S, b0[0] + bx, b0[1] + by, b0[2] + bz)] = 1u;
+    mark_block(B, block_of(S, b0[0] + bx, b0[1] + by, b0[2] + bz), si);
   }
 }
-// mark the (at most 8) blocks of one particle's stencil
-__device__ void flag_stencil_blocks(const ddm_scene &S, const int base[3], uint32_t *flags) {
+// mark the (at most 8) blocks of one stencil
+__device__ void flag_stencil_blocks(const ddm_batch &B, const ddm_scene &S, int si, const int base[3]) {
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int c = 0; c < 2; ++c)
-        flags[block_of(S, (base[0] + 2 * a) >> 2, (base[1] + 2 * b) >> 2, (base[2] + 2 * c) >> 2)] = 1u;
+        mark_block(B, block_of(S, (base[0] + 2 * a) >> 2, (base[1] + 2 * b) >> 2, (base[2] + 2 * c) >> 2), si);
 }
 
 // ---------------------------------------------------------------- P2G (+ F advection, SVD, return map)
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
         }
       }
     }
-    flag_tile_blocks(S, T, B.block_flags);
+    flag_tile_blocks(B, S, ch.x, T);
   } else {
     __syncthreads();  // publishes the staged records
     group_tasks<4>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
@@ -399,21 +407,12 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
     for (int g = threadIdx.x; g < G; g += kWG) {
       int b[3];
       unpack_base(gr.key[g], b);
-      flag_stencil_blocks(S, b, B.block_flags);
+      flag_stencil_blocks(B, S, ch.x, b);
     }
   }
 }
 
 // ---------------------------------------------------------------- grid helpers
-__device__ __forceinline__ int scene_of_block(const ddm_scene *sc, int n, int blk) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (sc[mid].blk_begin <= blk) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
 struct BlockCoord {
   int ix, iy, iz;
 };
@@ -451,9 +450,11 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
-  for (int blk = blockIdx.x * (kWG / 64) + wave; blk < B.n_blocks; blk += gridDim.x * (kWG / 64)) {
-    if (B.block_flags[blk] == 0u) continue;
-    const int si = scene_of_block(B.scenes, B.n_scenes, blk);
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const int2 entry = list[e];
+    const int blk = entry.x, si = entry.y;
     const ddm_scene &S = B.scenes[si];
     BlockCoord c = block_node(S, blk, lane);
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
@@ -486,9 +487,11 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
   float4 *dmmv = reinterpret_cast<float4 *>(B.grid_dmmv);
-  for (int blk = blockIdx.x * (kWG / 64) + wave; blk < B.n_blocks; blk += gridDim.x * (kWG / 64)) {
-    if (B.block_flags[blk] == 0u) continue;
-    const int si = scene_of_block(B.scenes, B.n_scenes, blk);
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const int2 entry = list[e];
+    const int blk = entry.x, si = entry.y;
     const ddm_scene &S = B.scenes[si];
     BlockCoord c = block_node(S, blk, lane);
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
@@ -581,6 +584,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *
 // ---------------------------------------------------------------- G2P
 __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                              const ddm_state *__restrict__ next_tab) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
@@ -685,6 +689,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
 __global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
@@ -724,7 +729,9 @@ inline int grid_wgs(const ddm_batch *b) {
 inline int check(const ddm_batch *b, const void *cur, const void *next) {
   if (!b || !cur || !next || b->n_chunks < 0 || b->n_blocks < 0 || b->n_scenes <= 0) return DDM_EINVAL;
   if (b->max_bodies > DDM_MAX_BODIES) return DDM_ETOOMANYBODIES;
-  if (!b->scenes || !b->chunks || !b->grid_mmv || !b->grid_v || !b->block_flags) return DDM_EINVAL;
+  if (!b->scenes || !b->chunks || !b->grid_mmv || !b->grid_v || !b->block_flags || !b->block_list ||
+      !b->block_count)
+    return DDM_EINVAL;
   return 0;
 }
 
@@ -803,5 +810,6 @@ DDM_API int ddm_batch_reset(const ddm_batch *b, hipStream_t stream) {
   if (b->grid_dv) k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_dv), nodes);
   if (b->grid_dmmv) k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_dmmv), nodes);
   k_zero_u32<<<wg, kWG, 0, stream>>>(b->block_flags, (size_t)b->n_blocks);
+  if (b->block_count) k_zero_u32<<<1, kWG, 0, stream>>>(reinterpret_cast<uint32_t *>(b->block_count), 1);
   return (int)hipGetLastError();
 }

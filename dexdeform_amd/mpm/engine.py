@@ -161,6 +161,8 @@ class Engine:
         self.grid_mmv = z(G, 4)
         self.grid_v = z(G, 4)
         self.block_flags = torch.zeros(self.n_blocks, dtype=torch.int32, device=dev)
+        self.block_list = torch.zeros(self.n_blocks, 2, dtype=torch.int32, device=dev)
+        self.block_count = torch.zeros(1, dtype=torch.int32, device=dev)
         if with_grad:
             self.grid_dv = z(G, 4)
             self.grid_dmmv = z(G, 4)
@@ -229,7 +231,7 @@ class Engine:
         self._batch = _lib.DdmBatch(
             len(self.scenes), self.n_chunks, self.n_blocks, max(s.n_bodies for s in self.scenes),
             p(self.scene_table), p(self.chunks), p(self.grid_mmv), p(self.grid_v), p(self.grid_dv), p(self.grid_dmmv),
-            p(self.block_flags))
+            p(self.block_flags), p(self.block_list), p(self.block_count))
 
     def table(self, x, v, F, C, body_pos, body_rot) -> torch.Tensor:
         """Per-scene ddm_state records for concatenated tensors (cached by address)."""
@@ -294,5 +296,5 @@ class Engine:
 
     def nbytes(self) -> int:
         ts = [self.mass, self.vol, self.mu_lam_yield, self.body_prim, self.body_args, self.grid_mmv, self.grid_v,
-              self.block_flags, self.grid_dv, self.grid_dmmv, self.tmp_Ft, self.tmp_U, self.tmp_V, self.tmp_sig]
+              self.block_flags, self.block_list, self.block_count, self.grid_dv, self.grid_dmmv, self.tmp_Ft, self.tmp_U, self.tmp_V, self.tmp_sig]
         return int(sum(t.numel() * t.element_size() for t in ts if t is not None))

@@ -10,6 +10,8 @@ Restates the reference's host-side kinematics without pytorch3d:
 """
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 import torch
 
@@ -179,6 +181,27 @@ class HingeTree:
         q = self.joint_targets(q0, action, substeps)
         pos, rot = self.fk(q)
         return pos, rot, q[-1]
+
+    def trajectory(self, q0: torch.Tensor, actions: torch.Tensor, substeps: int):
+        """Poses of every substep for a sequence of joint-velocity actions [E, n_act],
+        each env step starting from the joints the previous one ended at:
+        (pos [E*S+1, n, 3], rot [E*S+1, n, 4]); row 0 is the pose at q0."""
+        p0, r0 = self.fk(q0)
+        P, R = [p0[None]], [r0[None]]
+        q = q0
+        for a in actions:
+            p, r, q = self.poses(q, a, substeps)
+            P.append(p)
+            R.append(r)
+        return torch.cat(P), torch.cat(R)
+
+    def to(self, dtype=None, device=None) -> "HingeTree":
+        """Copy of the tree with its tensors converted (e.g. float64 for finite differences)."""
+        out = copy.copy(self)
+        for k in ("offsets", "axes", "geom", "lower", "upper", "base", "scale"):
+            setattr(out, k, getattr(self, k).to(dtype=dtype, device=device))
+        out.action_map = self.action_map.to(device=device)
+        return out
 
 
 def synthetic_hand(seed: int = 4, palm=(0.5, 0.34, 0.5), device="cpu"):

@@ -10,6 +10,11 @@ pass recomputes one K-substep segment at a time from its checkpoint into a K+1 s
 buffer and runs the adjoint through it. States then cost (T/K + K + 1) instead of T+1
 copies, at the price of one extra forward substep per substep.
 
+A whole forward + backward pass can be captured once into a HIP graph
+(:meth:`Rollout.capture`) and replayed (:meth:`Rollout.replay`): every buffer, table and
+launch of a rollout is fixed once the engine and horizon are, so a replay re-runs the
+same kernels on whatever the initial state, poses and loss seed hold at that time.
+
 Body poses of all substeps live in ``pose_pos [T+1, NB, 3]`` / ``pose_rot [T+1, NB, 4]``;
 the states' pose pointers alias those rows, so setting a trajectory's poses is one
 device copy and nothing crosses the host per substep. Pose gradients accumulate into
@@ -151,6 +156,33 @@ class Rollout:
 
     def initial_grad(self) -> _Particles:
         return self._g0
+
+    # ------------------------------------------------------------------ HIP graph
+    def capture(self, x_grad: torch.Tensor):
+        """Capture forward() + backward(x_grad=x_grad) into a HIP graph. x_grad is kept
+        and read at every replay, so the loss seed may change between replays; so may
+        the initial state and the poses. The graph holds only kernel launches and
+        device copies (grid and gradient buffers are cleared by kernels)."""
+        self._seed = x_grad
+        side = torch.cuda.Stream()  # default priority
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # populates the table caches: no uploads during capture
+            self.forward()
+            self.backward(x_grad=x_grad)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.forward()
+            self.backward(x_grad=x_grad)
+        self._graph = g
+        return g
+
+    def replay(self):
+        """One captured forward + backward pass (see capture)."""
+        if getattr(self, "_graph", None) is None:
+            raise RuntimeError("no captured graph: call capture() first")
+        self._graph.replay()
 
     def state_bytes(self) -> int:
         per = self.engine.n_particles_total * (3 + 3 + 9 + 9) * 4

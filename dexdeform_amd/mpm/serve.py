@@ -274,6 +274,8 @@ class RolloutServer:
         z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=self.device)
         self.grid_mmv, self.grid_v, self.grid_dv, self.grid_dmmv = z(G, 4), z(G, 4), z(G, 4), z(G, 4)
         self.block_flags = torch.zeros(arena_blocks, dtype=torch.int32, device=self.device)
+        self.block_list = torch.zeros(arena_blocks, 2, dtype=torch.int32, device=self.device)
+        self.block_count = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.memory_budget = memory_budget
         self.max_active = max_active
         self.waiting: List[Request] = []
@@ -336,7 +338,8 @@ class RolloutServer:
         tabs = [state_table(x, dev) for x in ((cur, nxt, gc, gn) if kind == "B" else (cur, nxt))]
         batch = _lib.DdmBatch(len(entries), len(rows), self.arena.n, max(int(r["n_bodies"]) for r in recs),
                               scene_tab.data_ptr(), chunks.data_ptr(), self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
-                              self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr())
+                              self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
+                              self.block_list.data_ptr(), self.block_count.data_ptr())
         import ctypes
         st = _lib.stream_ptr()
         if kind == "F":

@@ -153,3 +153,79 @@ def batched_scenes(n_scenes=64, seed=7, n_range=(5000, 60000), grid=64, dt=5e-5,
         sc.meta["seed"] = s
         out.append(sc)
     return out
+
+
+def served_stream(rate: float, n_requests: int, seed=13, n_range=(5000, 100000), grids=(48, 64, 96),
+                  horizon_range=(20, 200), checkpoint_every=10, dt=5e-5, ppc=6.0):
+    """Rollout requests of a Poisson arrival stream (``rate`` requests/s): particle counts
+    U[n_range] in a box sized for about ``ppc`` particles per cell, grid drawn from
+    ``grids``, horizon U[horizon_range] in multiples of ``checkpoint_every`` (the
+    checkpoint interval), 2-6 capsules with constant random velocities, yield stress
+    U[20, 100], an L2 target near the initial shape. Returns a list of dicts with keys
+    scene, pose_pos, pose_rot, target, arrival, checkpoint_every."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        grid = int(rng.choice(grids))
+        dx = 1.0 / grid
+        T = int(rng.integers(horizon_range[0] // checkpoint_every, horizon_range[1] // checkpoint_every + 1))
+        T *= checkpoint_every
+        side = min(0.6, dx * (n / ppc) ** (1.0 / 3.0))
+        c = np.array([0.5, 0.06 + side / 2, 0.5])
+        x0 = box_particles(n, c - side / 2, c + side / 2, int(rng.integers(1 << 30)), dx)
+        nb = int(rng.integers(2, 7))
+        pos, rot, args = random_capsules(rng, nb, [0.5, c[1] + side / 2 + 0.05, 0.5], spread=side / 2)
+        sc = make_scene(x0, grid, dx, dt, capsule_prims(nb), args, pos, rot,
+                        yield_stress=float(rng.uniform(20.0, 100.0)))
+        vel = rng.uniform(-0.5, 0.5, size=(nb, 3)).astype(np.float32)
+        steps = np.arange(T + 1, dtype=np.float32)[:, None, None] * np.float32(dt)
+        pose_pos = (sc.body_pos[None] + steps * vel[None]).astype(np.float32)
+        pose_rot = np.repeat(sc.body_rot[None], T + 1, 0)
+        out.append(dict(scene=sc, pose_pos=pose_pos, pose_rot=pose_rot,
+                        target=plain_target(x0, seed=int(rng.integers(1 << 30))), arrival=t,
+                        checkpoint_every=checkpoint_every))
+    return out
+
+
+def hand_scene(n_particles=50000, seed=3, grid=64, dt=5e-5, friction=0.9):
+    """Hand-contact scene: particles in a seeded ellipsoid (centre (0.5, 0.2, 0.5), radii
+    (0.12, 0.08, 0.10)) with the 22-link synthetic capsule hand (tree seed 4) reaching
+    into it from above. Returns (scene, hinge tree, initial joint angles); the scene's
+    body poses are the hand's pose at those angles."""
+    from .kinematics import synthetic_hand
+    tree, args, q0 = synthetic_hand(seed=4)
+    dx = 1.0 / grid
+    x0 = ellipsoid_particles(n_particles, [0.5, 0.2, 0.5], [0.12, 0.08, 0.10], seed, dx)
+    pos, rot = tree.fk(q0)
+    sc = make_scene(x0, grid, dx, dt, capsule_prims(len(args), friction), args, pos.numpy(), rot.numpy())
+    return sc, tree, q0
+
+
+def hand_actions(n_env_steps: int, n_joints=22, seed=4) -> np.ndarray:
+    """Seeded joint-velocity actions U(-1, 1) [n_env_steps, n_joints]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(-1.0, 1.0, size=(n_env_steps, n_joints)).astype(np.float32)
+
+
+def large_scene(n_particles=1_000_000, seed=6, grid=128, dt=5e-5, friction=0.9):
+    """Large single scene: a 128^3 grid, particles filling a union of 3 seeded boxes
+    (half-extents U[0.06, 0.14] x U[0.05, 0.10] x U[0.06, 0.14], resting near the floor,
+    centres within 0.15 of (0.5, 0.5)) under the 22-link capsule hand. Returns (scene,
+    hinge tree, initial joint angles)."""
+    from .kinematics import synthetic_hand
+    rng = np.random.Generator(np.random.PCG64(seed))
+    boxes = []
+    for _ in range(3):
+        half = rng.uniform([0.06, 0.05, 0.06], [0.14, 0.10, 0.14])
+        c = np.array([0.5 + rng.uniform(-0.15, 0.15), 0.04 + half[1], 0.5 + rng.uniform(-0.15, 0.15)])
+        boxes.append((c - half, c + half))
+    dx = 1.0 / grid
+    x0 = union_boxes_particles(n_particles, boxes, int(rng.integers(1 << 30)), dx)
+    tree, args, q0 = synthetic_hand(seed=4)
+    pos, rot = tree.fk(q0)
+    sc = make_scene(x0, grid, dx, dt, capsule_prims(len(args), friction), args, pos.numpy(), rot.numpy())
+    sc.meta["boxes"] = boxes
+    return sc, tree, q0

@@ -136,6 +136,9 @@ typedef struct ddm_batch {
   float *grid_dv;              /* [64*n_blocks][4] adjoint of grid_v; zero between substeps */
   float *grid_dmmv;            /* [64*n_blocks][4] adjoint of (m, mv) */
   uint32_t *block_flags;       /* [n_blocks]; zero between substeps */
+  int32_t *block_list;         /* [n_blocks][2] (block, scene index) of the blocks P2G flagged, in
+                                  flagging order; the grid kernels visit exactly these */
+  int32_t *block_count;        /* [1] entries in block_list; zero between substeps */
 } ddm_batch;
 
 /* Per-scene state record (device memory, one per scene, in scene order). */

@@ -1,0 +1,94 @@
+"""Large single scene on the GPU (not collected by pytest):
+
+    python tests/check_large_scene.py [--particles 1000000] [--env-steps 50] [--checkpoint 25]
+
+128^3 grid, 1,000,000 particles in a union of 3 seeded boxes, the 22-link capsule hand,
+50 env steps x 10 substeps = 500 substeps forward + backward with recompute checkpoints
+every 25 substeps. Reports throughput, peak device memory and a gradient check on 8 action
+coordinates as one JSON line.
+
+Gradient check: the FP64 oracle cannot run 16 rollouts of 500 substeps over 10^6 particles
+in reasonable time, so the check is against FP32 central differences of the GPU rollout
+itself (eps = 1e-2 in action units). Their rounding noise (FP32 state, atomic order) is
+reported beside the error: the same kernels are checked against FP64 finite differences
+at the smaller hand-contact size by tests/test_gpu_hand.py.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from dexdeform_amd.mpm import workloads as W  # noqa: E402
+from dexdeform_amd.mpm.engine import Engine  # noqa: E402
+from dexdeform_amd.mpm.rollout import Rollout  # noqa: E402
+from tests.helpers import rel_err  # noqa: E402
+
+SUB = 10
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--env-steps", type=int, default=50)
+    ap.add_argument("--checkpoint", type=int, default=25)
+    ap.add_argument("--coords", type=int, default=8)
+    ap.add_argument("--eps", type=float, default=1e-2)
+    a = ap.parse_args()
+    dev = torch.device("cuda:0")
+    sc, tree, q0 = W.large_scene(a.particles)
+    sc.spec.adjoint_mode = 1
+    T = a.env_steps * SUB
+    acts = W.hand_actions(a.env_steps)
+    target = torch.tensor(W.plain_target(sc.x0), device=dev)
+    eng = Engine([sc.spec], device=dev)
+    ro = Rollout(eng, T, checkpoint_every=a.checkpoint)
+    ro.set_initial(torch.tensor(sc.x0, device=dev))
+    treed, q0d = tree.to(device=dev), q0.to(dev)
+
+    def loss_of(act, grad=False):
+        A = torch.tensor(act, device=dev, requires_grad=grad)
+        P, R = treed.trajectory(q0d, A, SUB)
+        ro.set_poses(P.detach(), R.detach())
+        d = ro.forward().x - target
+        loss = (d.double() * d.double()).sum(-1).mean()
+        if not grad:
+            return float(loss.item())
+        ro.backward(x_grad=(2.0 / sc.n) * d)
+        ((P * ro.pos_grad).sum() + (R * ro.rot_grad).sum()).backward()
+        return float(loss.item()), A.grad.double().cpu().numpy()
+
+    loss_of(acts, grad=True)  # warm-up
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    loss, g = loss_of(acts, grad=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rng = np.random.default_rng(5)
+    coords = [(int(rng.integers(a.env_steps)), int(rng.integers(22))) for _ in range(a.coords)]
+    fd, noise = [], []
+    for (e, k) in coords:
+        ap_, am_ = acts.copy(), acts.copy()
+        ap_[e, k] += a.eps
+        am_[e, k] -= a.eps
+        lp, lm = loss_of(ap_), loss_of(am_)
+        fd.append((lp - lm) / (2 * a.eps))
+    l0 = [loss_of(acts) for _ in range(3)]  # run-to-run spread of the loss (atomic order)
+    gpick = np.array([g[e, k] for e, k in coords])
+    out = {"particles": sc.n, "grid": 128, "substeps": T, "checkpoint_every": a.checkpoint, "bodies": sc.nb,
+           "loss": loss, "fwd_bwd_seconds": el, "particle_substeps_per_s": sc.n * T / el,
+           "peak_device_bytes": int(torch.cuda.max_memory_allocated()),
+           "state_bytes": ro.state_bytes(), "coords": coords, "grad": gpick.tolist(), "fd_fp32": fd,
+           "grad_vs_fd_fp32": rel_err(gpick, np.array(fd)),
+           "loss_run_to_run_spread": float(np.ptp(l0)), "fd_noise_estimate": float(np.ptp(l0) / (2 * a.eps))}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,6 @@ def test_struct_layouts_match_header():
     from dexdeform_amd import _lib
     assert ctypes.sizeof(_lib.DdmScene) == 136
     assert ctypes.sizeof(_lib.DdmState) == 6 * 8
-    assert ctypes.sizeof(_lib.DdmBatch) == 16 + 7 * 8
+    assert ctypes.sizeof(_lib.DdmBatch) == 16 + 9 * 8
     hdr = open(HEADER).read()
     assert "} ddm_scene;                   /* 136 bytes */" in hdr
