@@ -100,7 +100,8 @@ def serve_benchmark(dev, rates, n_requests, seed):
         st = latency_stats(res)
         ps = sum(r.n * r.T for r in reqs)
         out.append({"rate_req_s": rate, "requests": n_requests, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                    "throughput": ps / wall, "wall_s": wall, "ticks": srv.ticks})
+                    "throughput": ps / wall, "wall_s": wall, "ticks": srv.ticks,
+                    "host_s": {k: round(v, 4) for k, v in srv.host_s.items()}})
         log(f"serve rate {rate}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
             f"throughput {ps / wall:.3e} particle-substeps/s")
         del srv, res

@@ -16,12 +16,16 @@ Requests of different particle count, grid size and horizon share the GPU:
   substeps run in one fused adjoint launch sequence (``ddm_substep_forward`` /
   ``ddm_substep_backward`` with per-scene state tables).
 * state stays on the device for the whole request; only the small scene/chunk/state
-  tables are uploaded per tick.
+  tables are uploaded per launch, from a pinned host ring with asynchronous copies, so
+  the host prepares the next tick while the GPU runs the current one (a synchronous
+  pageable upload would wait for all queued kernels first).
 
 Latency of a request = completion (an event recorded after its last operation) minus
 arrival; throughput = particle-substeps (forward + adjoint) per second of wall time.
 """
 from __future__ import annotations
+
+import ctypes
 
 import heapq
 import time
@@ -32,7 +36,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .engine import CHUNK, SceneSpec, chunk_table, state_table
+from .engine import CHUNK, SceneSpec, chunk_table
 
 
 @dataclass
@@ -105,12 +109,14 @@ class BlockArena:
         return sum(l for _, l in self.free)
 
 
-class _Particles:
-    def __init__(self, n, device, identity=True):
-        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=device)
-        self.x, self.v, self.F, self.C = z(n, 3), z(n, 3), z(n, 3, 3), z(n, 3, 3)
-        if identity and n:
-            self.F[:] = torch.eye(3, device=device)
+class _View:
+    """x, v, F, C views of one particle-state set inside a request's state block."""
+
+    def __init__(self, flat, n):
+        self.x = flat[0:3 * n].view(n, 3)
+        self.v = flat[3 * n:6 * n].view(n, 3)
+        self.F = flat[6 * n:15 * n].view(n, 3, 3)
+        self.C = flat[15 * n:24 * n].view(n, 3, 3)
 
     def tensors(self):
         return (self.x, self.v, self.F, self.C)
@@ -122,6 +128,70 @@ class _Particles:
     def zero_(self):
         for a in self.tensors():
             a.zero_()
+
+
+def _state_sets(n, count, device):
+    """count particle-state sets in one uninitialised allocation (every set is written
+    before it is read: by a substep, a checkpoint copy or an explicit initialisation)."""
+    block = torch.empty(count, 24 * n, dtype=torch.float32, device=device)
+    return [_View(block[i], n) for i in range(count)]
+
+
+def _to_device(a, device, dtype=np.float32) -> torch.Tensor:
+    """Asynchronous upload through freshly allocated pinned memory (the caching host
+    allocator keeps it until the copy has run)."""
+    h = torch.from_numpy(np.ascontiguousarray(a, dtype)).pin_memory()
+    return h.to(device, non_blocking=True)
+
+
+class _Staging:
+    """Per-launch tables: packed into one slot of a pinned host ring, copied
+    asynchronously into one device buffer on the launch stream. Launches on the same
+    stream consume the device buffer in order, so one device buffer suffices; a host
+    slot is reused only after the copy that read it has completed."""
+
+    ALIGN = 256
+
+    def __init__(self, device, nbytes=1 << 20, slots=8):
+        self.device = device
+        self.slots = slots
+        self._alloc(nbytes)
+
+    def _alloc(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.dev = torch.empty(self.nbytes, dtype=torch.uint8, device=self.device)
+        self.host = [torch.empty(self.nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(self.slots)]
+        self.events = [None] * self.slots
+        self.i = 0
+
+    def upload(self, parts):
+        """parts: numpy arrays; returns their device addresses (valid until the next
+        upload's copy runs, i.e. for the launch that follows on the same stream)."""
+        sizes = [int(a.nbytes) for a in parts]
+        need = sum((n + self.ALIGN - 1) // self.ALIGN * self.ALIGN for n in sizes)
+        if need > self.nbytes:
+            torch.cuda.synchronize()
+            self._alloc(max(need, 2 * self.nbytes))
+        slot = self.i
+        self.i = (self.i + 1) % self.slots
+        if self.events[slot] is not None:
+            self.events[slot].synchronize()
+        h = self.host[slot].numpy()
+        base = self.dev.data_ptr()
+        off, addrs = 0, []
+        for a, n in zip(parts, sizes):
+            h[off:off + n] = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+            addrs.append(base + off)
+            off += (n + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        self.dev[:off].copy_(self.host[slot][:off], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self.events[slot] = ev
+        return addrs
+
+
+def _state_records(records) -> np.ndarray:
+    return np.array([tuple(int(x) for x in r) for r in records], dtype=_lib.STATE_DTYPE)
 
 
 def request_bytes(req: Request) -> int:
@@ -153,12 +223,13 @@ class _Active:
         self.K = req.checkpoint_every or T
         if T % self.K:
             raise ValueError("checkpoint_every must divide the horizon")
-        f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+        f = lambda a: _to_device(a, dev)
         self.mass, self.vol, self.mly = f(sp.mass), f(sp.vol), f(np.asarray(sp.mu_lam_yield).reshape(-1, 3))
         self.prim = f(np.asarray(sp.body_prim).reshape(-1, 4)) if nb else None
         self.args = f(np.asarray(sp.body_args).reshape(-1, 4)) if nb else None
-        self.tmp = [torch.zeros(n, 9, device=dev), torch.zeros(n, 9, device=dev), torch.zeros(n, 9, device=dev),
-                    torch.zeros(n, 3, device=dev)]
+        tmp = torch.empty(30 * n, dtype=torch.float32, device=dev)  # written by the adjoint recompute
+        self.tmp = [tmp[0:9 * n].view(n, 9), tmp[9 * n:18 * n].view(n, 9), tmp[18 * n:27 * n].view(n, 9),
+                    tmp[27 * n:30 * n].view(n, 3)]
         nb_rows = max(nb, 1)
         self.pose_pos = torch.zeros(T + 1, nb_rows, 3, device=dev)
         self.pose_rot = torch.zeros(T + 1, nb_rows, 4, device=dev)
@@ -169,10 +240,17 @@ class _Active:
         self.pos_grad = torch.zeros_like(self.pose_pos)
         self.rot_grad = torch.zeros_like(self.pose_rot)
         self.full = self.K == T
-        self.ckpt = [] if self.full else [_Particles(n, dev) for _ in range(T // self.K + 1)]
-        self.seg = [_Particles(n, dev) for _ in range(self.K + 1)]
-        self.g = [_Particles(n, dev, identity=False) for _ in range(2)]
-        self.seg[0].x.copy_(f(req.x0))
+        n_ckpt = 0 if self.full else T // self.K + 1
+        sets = _state_sets(n, n_ckpt + self.K + 1 + 2, dev)
+        self.ckpt = sets[:n_ckpt]
+        self.seg = sets[n_ckpt:n_ckpt + self.K + 1]
+        self.g = sets[n_ckpt + self.K + 1:]
+        s0 = self.seg[0]
+        s0.x.copy_(f(req.x0))
+        s0.v.zero_()
+        s0.C.zero_()
+        s0.F.zero_()
+        s0.F.diagonal(dim1=1, dim2=2).fill_(1.0)
         if not self.full:
             self.ckpt[0].copy_(self.seg[0])
         self.target = f(req.target)
@@ -191,7 +269,7 @@ class _Active:
         r["body_prim"] = self.prim.data_ptr() if nb else 0
         r["body_args"] = self.args.data_ptr() if nb else 0
         r["tmp_Ft"], r["tmp_U"], r["tmp_V"], r["tmp_sig"] = (t.data_ptr() for t in self.tmp)
-        self.chunks = chunk_table([n])[:, 1:3]  # local (first, end) rows
+        self.chunks = chunk_table([n]).astype(np.int32)  # (0, first, end, 0) rows; column 0 = scene slot
         self.ops = self._schedule()
         self.pc = 0
         self.done_event = None
@@ -282,6 +360,8 @@ class RolloutServer:
         self.active: List[_Active] = []
         self.results: Dict[int, Result] = {}
         self.used_bytes = 0
+        self.staging = _Staging(self.device)
+        self.host_s = {"admit": 0.0, "launch": 0.0, "tick": 0.0}  # host seconds spent per phase
         self.t0 = None
         self.ticks = 0
         self.particle_substeps = 0
@@ -314,47 +394,41 @@ class RolloutServer:
                 break
             self.waiting.pop(0)
             self.used_bytes += need
+            t0 = time.perf_counter()
             self.active.append(_Active(self, req, b0, now))
+            self.host_s["admit"] += time.perf_counter() - t0
 
     # ------------------------------------------------------------------ launches
     def _launch(self, entries, kind):
         """entries: list of (active, op). One fused launch sequence for all of them."""
-        entries.sort(key=lambda e: e[0].blk_begin)  # the grid kernels look scenes up by block
-        recs = np.array([e[0].record for e in entries], dtype=_lib.SCENE_DTYPE)
-        rows = []
-        cur, nxt, gc, gn = [], [], [], []
-        for si, (a, op) in enumerate(entries):
-            for (b, e) in a.chunks:
-                rows.append((si, int(b), int(e), 0))
-            j, t = op[1], op[2]
-            cur.append(a.state_record(j, t))
-            nxt.append(a.state_record(j + 1, t + 1))
-            if kind == "B":
-                gc.append(a.grad_record(op[3], t))
-                gn.append(a.grad_record(op[4], t + 1))
-        dev = self.device
-        scene_tab = torch.as_tensor(recs.view(np.uint8).copy(), device=dev)
-        chunks = torch.as_tensor(np.asarray(rows, np.int32), device=dev)
-        tabs = [state_table(x, dev) for x in ((cur, nxt, gc, gn) if kind == "B" else (cur, nxt))]
-        batch = _lib.DdmBatch(len(entries), len(rows), self.arena.n, max(int(r["n_bodies"]) for r in recs),
-                              scene_tab.data_ptr(), chunks.data_ptr(), self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
+        recs = np.array([a.record for a, _ in entries], dtype=_lib.SCENE_DTYPE)
+        chunks = np.concatenate([a.chunks for a, _ in entries])
+        k = 0
+        for si, (a, _) in enumerate(entries):
+            chunks[k:k + len(a.chunks), 0] = si
+            k += len(a.chunks)
+        cur = [a.state_record(op[1], op[2]) for a, op in entries]
+        nxt = [a.state_record(op[1] + 1, op[2] + 1) for a, op in entries]
+        parts = [recs.view(np.uint8), chunks, _state_records(cur), _state_records(nxt)]
+        if kind == "B":
+            parts += [_state_records([a.grad_record(op[3], op[2]) for a, op in entries]),
+                      _state_records([a.grad_record(op[4], op[2] + 1) for a, op in entries])]
+        addr = self.staging.upload(parts)
+        batch = _lib.DdmBatch(len(entries), len(chunks), self.arena.n, max(int(r["n_bodies"]) for r in recs),
+                              addr[0], addr[1], self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
                               self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
                               self.block_list.data_ptr(), self.block_count.data_ptr())
-        import ctypes
         st = _lib.stream_ptr()
         if kind == "F":
-            _lib.check(self.lib.ddm_substep_forward(ctypes.byref(batch), tabs[0].data_ptr(), tabs[1].data_ptr(), st),
-                       "ddm_substep_forward")
+            _lib.check(self.lib.ddm_substep_forward(ctypes.byref(batch), addr[2], addr[3], st), "ddm_substep_forward")
         else:
-            _lib.check(self.lib.ddm_substep_backward(ctypes.byref(batch), tabs[0].data_ptr(), tabs[1].data_ptr(),
-                                                     tabs[2].data_ptr(), tabs[3].data_ptr(), st),
+            _lib.check(self.lib.ddm_substep_backward(ctypes.byref(batch), addr[2], addr[3], addr[4], addr[5], st),
                        "ddm_substep_backward")
-        # keep the tables alive until the stream has consumed them
-        self._keep.append((scene_tab, chunks, tabs))
         for a, _ in entries:
             self.particle_substeps += a.req.n if kind == "B" else 0
 
     def tick(self):
+        t_tick = time.perf_counter()
         now = self._now()
         self._admit(now)
         fwd, bwd = [], []
@@ -369,18 +443,20 @@ class RolloutServer:
                 finished.append(a)
                 continue
             (fwd if op[0] == "F" else bwd).append((a, op))
-        self._keep = []
+        t0 = time.perf_counter()
         if fwd:
             self._launch(fwd, "F")
         if bwd:
             for a, op in bwd:
                 a.g[op[3]].zero_()  # the adjoint substep adds into the state-t gradient
             self._launch(bwd, "B")
+        self.host_s["launch"] += time.perf_counter() - t0
         for a, _ in fwd + bwd:
             a.pc += 1
         if fwd or bwd:
             self.ticks += 1  # ticks that launched work
         self._collect()
+        self.host_s["tick"] += time.perf_counter() - t_tick
         return bool(fwd or bwd)
 
     def _collect(self, block=False):
@@ -404,7 +480,6 @@ class RolloutServer:
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
         self.t0 = time.perf_counter()
-        self._keep = []
         while self.waiting or self.active:
             if not self.active and self.waiting and self.waiting[0].arrival > self._now():
                 time.sleep(min(1e-3, self.waiting[0].arrival - self._now()))

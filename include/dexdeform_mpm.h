@@ -129,7 +129,7 @@ typedef struct ddm_batch {
   int32_t n_chunks;            /* particle chunks of <= 256 particles, none straddling scenes */
   int32_t n_blocks;            /* node blocks in the grid arenas */
   int32_t max_bodies;          /* max n_bodies over the scenes */
-  const ddm_scene *scenes;     /* device [n_scenes], sorted by blk_begin */
+  const ddm_scene *scenes;     /* device [n_scenes] */
   const int32_t *chunks;       /* device [n_chunks][4] = (scene, first, end, 0), local particle indices */
   float *grid_mmv;             /* [64*n_blocks][4] (m, mv); zero between substeps */
   float *grid_v;               /* [64*n_blocks][4] node velocity after the grid update */
