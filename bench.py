@@ -79,7 +79,7 @@ def build_rollout(scenes, poses, T, device, checkpoint_every=None):
     return eng, ro, gx
 
 
-def serve_benchmark(dev, rates, n_requests, seed):
+def serve_benchmark(dev, rates, seconds, seed):
     """Served rollouts: for each Poisson arrival rate, a stream of uneven requests
     (particles U[5k,100k], grid 48^3/64^3/96^3, horizon U[20,200] with checkpoints every
     10 substeps, forward + backward each) through the continuous-batching server.
@@ -90,6 +90,7 @@ def serve_benchmark(dev, rates, n_requests, seed):
     from dexdeform_amd.mpm.serve import Request, RolloutServer, latency_stats
     out = []
     for k, rate in enumerate(rates):
+        n_requests = max(24, int(round(rate * seconds)))
         stream = W.served_stream(rate, n_requests, seed=seed + 1000 * k)
         reqs = [Request(i, d["scene"].spec, d["scene"].x0, d["pose_pos"], d["pose_rot"], d["target"],
                         arrival=d["arrival"], checkpoint_every=d["checkpoint_every"]) for i, d in enumerate(stream)]
@@ -217,7 +218,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", default="10,50,100,200", help="request arrival rates (req/s) to sweep")
-    ap.add_argument("--serve-requests", type=int, default=24, help="requests per rate")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="arrival window per rate: rate x seconds requests (at least 24)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -301,7 +303,7 @@ def main():
 
     serve = None
     if not args.no_serve:
-        serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_requests, args.seed)
+        serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_seconds, args.seed)
 
     cpu = None
     if not args.no_cpu:

@@ -113,6 +113,7 @@ class _View:
     """x, v, F, C views of one particle-state set inside a request's state block."""
 
     def __init__(self, flat, n):
+        self.flat = flat  # contiguous x | v | F | C
         self.x = flat[0:3 * n].view(n, 3)
         self.v = flat[3 * n:6 * n].view(n, 3)
         self.F = flat[6 * n:15 * n].view(n, 3, 3)
@@ -122,12 +123,10 @@ class _View:
         return (self.x, self.v, self.F, self.C)
 
     def copy_(self, o):
-        for a, b in zip(self.tensors(), o.tensors()):
-            a.copy_(b)
+        self.flat.copy_(o.flat)  # one copy for the four arrays
 
     def zero_(self):
-        for a in self.tensors():
-            a.zero_()
+        self.flat.zero_()  # one fill kernel for the four arrays
 
 
 def _state_sets(n, count, device):
@@ -191,7 +190,8 @@ class _Staging:
 
 
 def _state_records(records) -> np.ndarray:
-    return np.array([tuple(int(x) for x in r) for r in records], dtype=_lib.STATE_DTYPE)
+    """ddm_state records (six device addresses each) from lists of ints."""
+    return np.array(records, dtype=np.int64).reshape(-1, 6)
 
 
 def request_bytes(req: Request) -> int:
@@ -271,6 +271,7 @@ class _Active:
         r["tmp_Ft"], r["tmp_U"], r["tmp_V"], r["tmp_sig"] = (t.data_ptr() for t in self.tmp)
         self.chunks = chunk_table([n]).astype(np.int32)  # (0, first, end, 0) rows; column 0 = scene slot
         self.ops = self._schedule()
+        self._addresses()
         self.pc = 0
         self.done_event = None
         self.loss = None
@@ -309,6 +310,10 @@ class _Active:
         x = self._final().x
         d = x - self.target
         self.loss = (d * d).sum(-1).mean()
+        # read back through pinned memory once the request's completion event has fired;
+        # .item() here would synchronize the whole stream
+        self.loss_host = torch.empty((), dtype=torch.float32, pin_memory=True)
+        self.loss_host.copy_(self.loss, non_blocking=True)
         g = self.g[0]
         g.zero_()
         g.x.copy_(2.0 / self.req.n * d)
@@ -316,15 +321,23 @@ class _Active:
         self.rot_grad.zero_()
         self.x_final = x.clone()
 
+    def _addresses(self):
+        """Device addresses behind the per-launch state records (fixed for the request)."""
+        self.seg_addr = [[int(t.data_ptr()) for t in s.tensors()] for s in self.seg]
+        self.g_addr = [[int(t.data_ptr()) for t in g.tensors()] for g in self.g]
+        rows = self.pose_pos.shape[1]
+        self.pose_addr = (int(self.pose_pos.data_ptr()), int(self.pose_rot.data_ptr()), rows * 12, rows * 16)
+        self.pgrad_addr = (int(self.pos_grad.data_ptr()), int(self.rot_grad.data_ptr()))
+        self.rec_bytes = np.array([self.record], dtype=_lib.SCENE_DTYPE).view(np.uint8)
+
     def state_record(self, j, t):
-        s = self.seg[j]
-        return (s.x.data_ptr(), s.v.data_ptr(), s.F.data_ptr(), s.C.data_ptr(), self.pose_pos[t].data_ptr(),
-                self.pose_rot[t].data_ptr())
+        pp, pr, sp, sr = self.pose_addr
+        return self.seg_addr[j] + [pp + t * sp, pr + t * sr]
 
     def grad_record(self, slot, t):
-        g = self.g[slot]
-        return (g.x.data_ptr(), g.v.data_ptr(), g.F.data_ptr(), g.C.data_ptr(), self.pos_grad[t].data_ptr(),
-                self.rot_grad[t].data_ptr())
+        _, _, sp, sr = self.pose_addr
+        gp, gr = self.pgrad_addr
+        return self.g_addr[slot] + [gp + t * sp, gr + t * sr]
 
     def next_kernel_op(self):
         """Run host-side pre/post operations up to the next kernel operation and return
@@ -401,7 +414,7 @@ class RolloutServer:
     # ------------------------------------------------------------------ launches
     def _launch(self, entries, kind):
         """entries: list of (active, op). One fused launch sequence for all of them."""
-        recs = np.array([a.record for a, _ in entries], dtype=_lib.SCENE_DTYPE)
+        recs = np.concatenate([a.rec_bytes for a, _ in entries])
         chunks = np.concatenate([a.chunks for a, _ in entries])
         k = 0
         for si, (a, _) in enumerate(entries):
@@ -409,12 +422,14 @@ class RolloutServer:
             k += len(a.chunks)
         cur = [a.state_record(op[1], op[2]) for a, op in entries]
         nxt = [a.state_record(op[1] + 1, op[2] + 1) for a, op in entries]
-        parts = [recs.view(np.uint8), chunks, _state_records(cur), _state_records(nxt)]
+        parts = [recs, chunks, _state_records(cur), _state_records(nxt)]
         if kind == "B":
             parts += [_state_records([a.grad_record(op[3], op[2]) for a, op in entries]),
                       _state_records([a.grad_record(op[4], op[2] + 1) for a, op in entries])]
         addr = self.staging.upload(parts)
-        batch = _lib.DdmBatch(len(entries), len(chunks), self.arena.n, max(int(r["n_bodies"]) for r in recs),
+        # blocks addressable by this launch: the grid kernels size their launch by it
+        n_blocks = max(a.blk_begin + a.n_blocks for a, _ in entries)
+        batch = _lib.DdmBatch(len(entries), len(chunks), n_blocks, max(a.req.spec.n_bodies for a, _ in entries),
                               addr[0], addr[1], self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
                               self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
                               self.block_list.data_ptr(), self.block_count.data_ptr())
@@ -466,8 +481,8 @@ class RolloutServer:
                 a.done_event.synchronize()
                 fin = self._now()
                 g0 = a.g[a.final_grad_slot]
-                self.results[a.req.rid] = Result(a.req.rid, float(a.loss.item()), a.x_final, a.pos_grad, a.rot_grad,
-                                                 g0.x.clone(), a.req.arrival, a.admitted, fin)
+                self.results[a.req.rid] = Result(a.req.rid, float(a.loss_host.item()), a.x_final, a.pos_grad,
+                                                 a.rot_grad, g0.x.clone(), a.req.arrival, a.admitted, fin)
                 self.used_bytes -= request_bytes(a.req)
                 self.arena.release(a.blk_begin, a.n_blocks)
             else:

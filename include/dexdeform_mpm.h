@@ -127,7 +127,9 @@ typedef struct ddm_scene {
 typedef struct ddm_batch {
   int32_t n_scenes;
   int32_t n_chunks;            /* particle chunks of <= 256 particles, none straddling scenes */
-  int32_t n_blocks;            /* node blocks in the grid arenas */
+  int32_t n_blocks;            /* node blocks addressable by this launch (every scene's
+                                  blk_begin + n_blocks <= n_blocks <= arena size); sizes
+                                  the grid kernels' launch */
   int32_t max_bodies;          /* max n_bodies over the scenes */
   const ddm_scene *scenes;     /* device [n_scenes] */
   const int32_t *chunks;       /* device [n_chunks][4] = (scene, first, end, 0), local particle indices */
