@@ -40,6 +40,14 @@ KERNEL_BYTES = {
     "grid_adj": 0,
     "p2g_adj": 464,      # read x,v,F,C,material,temps,dF' (272) ; read+write dx,dv,dF,dC (192)
 }
+# HBM bytes per particle per launch measured on the default workload (64 scenes, seed 7)
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
+# for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0). Re-measured whenever a
+# kernel changes; reported as roofline.traffic for the default workload only.
+TRAFFIC_BYTES = {
+    "p2g_fwd": 223, "grid_fwd": 10, "g2p_fwd": 88, "p2g_recompute": 308, "grid_recompute": 7, "g2p_adj": 175,
+    "grid_adj": 17, "p2g_adj": 487,
+}
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
 
@@ -333,7 +341,9 @@ def main():
         "kernel_ms_per_launch": kt,
         "algorithmic_fwd_bwd_GBps": alg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_BYTES[dom] * n_total
+                                 if (args.scenes, args.seed) == (64, 7) else None)},
         "serve": serve,
         "cpu_baseline": cpu,
     }

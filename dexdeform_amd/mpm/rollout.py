@@ -31,22 +31,26 @@ from .engine import Engine
 
 
 class _Particles:
+    """x, v, F, C of one particle state in one contiguous allocation, so that a clear or
+    a copy of the whole state is a single kernel."""
+
     def __init__(self, n, device, identity=True):
-        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=device)
-        self.x, self.v, self.F, self.C = z(n, 3), z(n, 3), z(n, 3, 3), z(n, 3, 3)
+        self.flat = torch.zeros(24 * n, dtype=torch.float32, device=device)
+        self.x = self.flat[0:3 * n].view(n, 3)
+        self.v = self.flat[3 * n:6 * n].view(n, 3)
+        self.F = self.flat[6 * n:15 * n].view(n, 3, 3)
+        self.C = self.flat[15 * n:24 * n].view(n, 3, 3)
         if identity and n:
-            self.F[:] = torch.eye(3, device=device)
+            self.F.diagonal(dim1=1, dim2=2).fill_(1.0)
 
     def tensors(self):
         return (self.x, self.v, self.F, self.C)
 
     def copy_(self, o):
-        for a, b in zip(self.tensors(), o.tensors()):
-            a.copy_(b)
+        self.flat.copy_(o.flat)
 
     def zero_(self):
-        for a in self.tensors():
-            a.zero_()
+        self.flat.zero_()
 
 
 class Rollout:
