@@ -7,7 +7,9 @@ the single-scene path.
 Workload (default, "batched"): 64 scenes, particle counts U[5k, 60k], 64^3 grid each,
 yield stress U[20, 100], 2-22 capsules per scene with constant random velocities,
 100 substeps forward then 100 adjoint substeps, all scenes in one launch per stage,
-the whole rollout captured once into a HIP graph and replayed. A step = one such
+the whole rollout captured once into a HIP graph and replayed. The adjoint reads the
+forward pass's tape (SVD temporaries and grid snapshots) instead of recomputing P2G and
+the grid update; the recomputing adjoint (the reference's scheme) is measured too. A step = one such
 rollout (forward + backward) of all 64 scenes. Also reported: the same rollout with
 eager launches, and the scenes run one at a time (plain; eager and graph-replayed).
 
@@ -47,9 +49,17 @@ KERNEL_BYTES = {
 TRAFFIC_BYTES = {
     "p2g_fwd": 223, "grid_fwd": 10, "g2p_fwd": 88, "p2g_recompute": 308, "grid_recompute": 7, "g2p_adj": 175,
     "grid_adj": 17, "p2g_adj": 487,
+    # taped rollout
+    "taped_p2g_fwd": 344, "taped_grid_fwd": 17, "taped_g2p_fwd": 88, "taped_restore": 13, "taped_g2p_adj": 175,
+    "taped_grid_adj": 17, "taped_p2g_adj": 487,
 }
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
+# the taped rollout: its forward P2G also writes the temporaries; its adjoint restores the
+# grid snapshot (no recompute unless the snapshot overflowed)
+KERNEL_BYTES_TAPED = dict(KERNEL_BYTES, p2g_fwd=152 + 120, grid_fwd=0, restore=0)
+FWD_NAMES_TAPED = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
+BWD_NAMES_TAPED = ["restore", "g2p_adj", "grid_adj", "p2g_adj"]
 
 
 def log(*a):
@@ -70,12 +80,12 @@ def make_batched(n_scenes, seed, T):
     return scenes, poses
 
 
-def build_rollout(scenes, poses, T, device, checkpoint_every=None):
+def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False):
     import torch
     from dexdeform_amd.mpm.engine import Engine
     from dexdeform_amd.mpm.rollout import Rollout
     eng = Engine([s.spec for s in scenes], device=device, with_grad=True)
-    ro = Rollout(eng, T, checkpoint_every=checkpoint_every)
+    ro = Rollout(eng, T, checkpoint_every=checkpoint_every, tape=tape)
     x0 = torch.tensor(np.concatenate([s.x0 for s in scenes]), device=device)
     ro.set_initial(x0)
     if eng.n_bodies_total:
@@ -150,10 +160,18 @@ class KernelTimer:
             return evs, arr
 
         assert ro.full, "kernel timing uses the non-checkpointed rollout"
+        taped = ro.tape is not None
+        fwd_names, bwd_names = (FWD_NAMES_TAPED, BWD_NAMES_TAPED) if taped else (FWD_NAMES, BWD_NAMES)
+        batches = [eng._batch_with_scenes(ro.tape.scene_tabs[t]) for t in range(ro.T)] if taped else None
         for t in range(ro.T):
             evs, arr = events(4)
-            _lib.check(lib.ddm_substep_forward_timed(ctypes.byref(eng._batch), ro._sview(t, t).data_ptr(),
-                                                     ro._sview(t + 1, t + 1).data_ptr(), st, arr))
+            if taped:
+                _lib.check(lib.ddm_substep_forward_taped_timed(
+                    ctypes.byref(batches[t]), ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(),
+                    ctypes.byref(ro.tape.tapes[t]), st, arr))
+            else:
+                _lib.check(lib.ddm_substep_forward_timed(ctypes.byref(eng._batch), ro._sview(t, t).data_ptr(),
+                                                         ro._sview(t + 1, t + 1).data_ptr(), st, arr))
             fwd_ev.append((evs, arr))
         gn = ro.g[0]
         gn.x.copy_(gx)
@@ -166,18 +184,23 @@ class KernelTimer:
         for t in range(ro.T - 1, -1, -1):
             nxt, cur = cur, 1 - cur
             ro.g[cur].zero_()
-            evs, arr = events(6)
-            _lib.check(lib.ddm_substep_backward_timed(
-                ctypes.byref(eng._batch), ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(),
-                ro._gview(cur, t).data_ptr(), ro._gview(nxt, t + 1).data_ptr(), st, arr))
+            tabs = (ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(), ro._gview(cur, t).data_ptr(),
+                    ro._gview(nxt, t + 1).data_ptr())
+            if taped:
+                evs, arr = events(5)
+                _lib.check(lib.ddm_substep_backward_taped_timed(ctypes.byref(batches[t]), *tabs,
+                                                                ctypes.byref(ro.tape.tapes[t]), st, arr))
+            else:
+                evs, arr = events(6)
+                _lib.check(lib.ddm_substep_backward_timed(ctypes.byref(eng._batch), *tabs, st, arr))
             bwd_ev.append((evs, arr))
         torch.cuda.synchronize()
-        tot = {k: 0.0 for k in FWD_NAMES + BWD_NAMES}
+        tot = {k: 0.0 for k in fwd_names + bwd_names}
         for evs, _ in fwd_ev:
-            for i, k in enumerate(FWD_NAMES):
+            for i, k in enumerate(fwd_names):
                 tot[k] += evs[i].elapsed_time(evs[i + 1])
         for evs, _ in bwd_ev:
-            for i, k in enumerate(BWD_NAMES):
+            for i, k in enumerate(bwd_names):
                 tot[k] += evs[i].elapsed_time(evs[i + 1])
         return {k: v / ro.T for k, v in tot.items()}  # ms per launch
 
@@ -225,6 +248,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tape", action="store_true",
+                    help="adjoint recomputes P2G and the grid update (as the reference does) instead of "
+                         "reading the forward pass's tape")
+    ap.add_argument("--no-recompute", action="store_true",
+                    help="skip the extra measurement of the recompute adjoint")
     ap.add_argument("--serve-rates", default="10,50,100,200", help="request arrival rates (req/s) to sweep")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival window per rate: rate x seconds requests (at least 24)")
@@ -251,7 +279,8 @@ def main():
     log(f"batched: {len(scenes)} scenes, {n_total} particles, {sum(s.nb for s in scenes)} bodies, T={T}")
 
     # ---------------- batched: eager launches, then the same rollout replayed from a HIP graph
-    eng, ro, gx = build_rollout(scenes, poses, T, dev)
+    tape = not args.no_tape
+    eng, ro, gx = build_rollout(scenes, poses, T, dev, tape=tape)
     for _ in range(args.warmup):
         run_step(ro, gx)
     torch.cuda.synchronize()
@@ -274,9 +303,29 @@ def main():
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = n_total * T / ((t1 - t0) / args.steps)
+    if tape:
+        log(f"taped adjoint: {ro.tape_recomputed()} substeps of the last segment recomputed; "
+            f"tape {ro.tape.nbytes() / 2**30:.1f} GiB")
+    recompute_value = None
+    if tape and not args.no_recompute:
+        # the same rollout with the reference's recompute-in-backward adjoint
+        del ro._graph
+        _, rr, gr_ = build_rollout(scenes, poses, T, dev)
+        rr.capture(gr_)
+        rr.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            rr.replay()
+            torch.cuda.synchronize()
+        recompute_value = n_total * T / ((time.perf_counter() - t0) / args.steps)
+        log(f"batched graph, recompute adjoint: {recompute_value:.3e} particle-substeps/s")
+        del rr, gr_
+        torch.cuda.empty_cache()
     kt = KernelTimer(ro).timed_step(gx)
     log("per-kernel ms/launch: " + ", ".join(f"{k}={v:.4f}" for k, v in kt.items()))
-    bytes_per = {k: KERNEL_BYTES[k] * n_total for k in kt}
+    kb = KERNEL_BYTES_TAPED if tape else KERNEL_BYTES
+    bytes_per = {k: kb[k] * n_total for k in kt}
     dom = max(kt, key=kt.get)
     achieved = bytes_per[dom] / (kt[dom] * 1e-3) / 1e9 if kt[dom] > 0 else 0.0
     sum_ms = sum(kt.values())
@@ -288,7 +337,7 @@ def main():
     if not args.no_plain:
         tot_e, tot_g, tot_ps = 0.0, 0.0, 0
         for i, (sc, po) in enumerate(zip(scenes, poses)):
-            e1, r1, g1 = build_rollout([sc], [po], T, dev)
+            e1, r1, g1 = build_rollout([sc], [po], T, dev, tape=tape)
             run_step(r1, g1)
             torch.cuda.synchronize()
             a = time.perf_counter()
@@ -334,6 +383,8 @@ def main():
         "config": {"workload": f"batched uneven scenes: {len(scenes)} scenes, particles U[5k,60k] "
                                f"(total {n_total}), 64^3 grid each, 2-22 capsules, {T} substeps fwd+bwd",
                    "global_batch": len(scenes), "substeps": T, "particles": n_total, "parallelism": "none"},
+        "adjoint": "taped" if tape else "recompute",
+        "recompute_value": recompute_value,
         "eager_value": eager_value,
         "plain_value": plain_value,
         "plain_eager_value": plain_eager_value,
@@ -342,8 +393,9 @@ def main():
         "algorithmic_fwd_bwd_GBps": alg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_BYTES[dom] * n_total
-                                 if (args.scenes, args.seed) == (64, 7) else None)},
+                     "traffic": (TRAFFIC_BYTES[("taped_" if tape else "") + dom] * n_total
+                                 if (args.scenes, args.seed) == (64, 7) and
+                                 ("taped_" if tape else "") + dom in TRAFFIC_BYTES else None)},
         "serve": serve,
         "cpu_baseline": cpu,
     }

@@ -58,6 +58,14 @@ class DdmBatch(ctypes.Structure):
     ]
 
 
+class DdmGridTape(ctypes.Structure):
+    _fields_ = [("grid_mmv", c_void_p), ("grid_v", c_void_p), ("block_list", c_void_p), ("status", c_void_p),
+                ("capacity", c_int32), ("reserved", c_int32)]
+
+
+assert ctypes.sizeof(DdmGridTape) == 40
+
+
 class DdmState(ctypes.Structure):
     _fields_ = [("x", c_void_p), ("v", c_void_p), ("F", c_void_p), ("C", c_void_p),
                 ("body_pos", c_void_p), ("body_rot", c_void_p)]
@@ -72,7 +80,9 @@ assert STATE_DTYPE.itemsize == ctypes.sizeof(DdmState) == 48
 REFERENCE_SYMBOLS = ["compute_svd", "compute_svd_grad", "p2g", "p2g_grad", "grid_op_v2", "grid_op_v2_grad", "g2p",
                      "g2p_grad", "compute_dist", "particle2mass"]
 FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_forward_timed",
-                 "ddm_substep_backward_timed", "ddm_batch_reset", "ddm_version"]
+                 "ddm_substep_backward_timed", "ddm_substep_forward_taped", "ddm_substep_backward_taped",
+                 "ddm_substep_forward_taped_timed", "ddm_substep_backward_taped_timed", "ddm_batch_reset",
+                 "ddm_version"]
 ALL_SYMBOLS = REFERENCE_SYMBOLS + FUSED_SYMBOLS
 
 _P = c_void_p
@@ -94,6 +104,11 @@ _ARGTYPES = {
     "ddm_substep_backward": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P],
     "ddm_substep_forward_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P],
     "ddm_substep_backward_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
+    "ddm_substep_forward_taped": [ctypes.POINTER(DdmBatch), _P, _P, ctypes.POINTER(DdmGridTape), _P],
+    "ddm_substep_backward_taped": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, ctypes.POINTER(DdmGridTape), _P],
+    "ddm_substep_forward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, ctypes.POINTER(DdmGridTape), _P, _P],
+    "ddm_substep_backward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, ctypes.POINTER(DdmGridTape), _P,
+                                         _P],
     "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],
     "ddm_version": [],
 }

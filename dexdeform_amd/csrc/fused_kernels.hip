@@ -312,11 +312,16 @@ __device__ void flag_stencil_blocks(const ddm_batch &B, const ddm_scene &S, int 
 }
 
 // ---------------------------------------------------------------- P2G (+ F advection, SVD, return map)
-// BWD=false: forward substep, writes next.F.
-// BWD=true : adjoint recompute, writes the F_tmp/U/V/sigma temporaries instead.
-template <bool BWD>
+enum P2GMode { kForward = 0, kRecompute = 1, kForwardTaped = 2 };
+// kForward     : forward substep, writes next.F.
+// kRecompute   : adjoint recompute, writes the F_tmp/U/V/sigma temporaries instead.
+// kForwardTaped: writes both (the taped adjoint reads the temporaries).
+// need (optional): the kernel runs only if *need != 0 (a taped adjoint that has to
+// recompute decides this on the device).
+template <int MODE>
 __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                             const ddm_state *__restrict__ next_tab) {
+                                             const ddm_state *__restrict__ next_tab, const int32_t *need) {
+  if (need && *need == 0) return;  // uniform over the grid
   __shared__ ScatterLDS<4, kTileP2G> sh;
   __shared__ StencilRecs rc;
   __shared__ Groups gr;
@@ -339,14 +344,13 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
     m33 Fn;
     p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                  ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
-    if (BWD) {
+    if (MODE != kForward) {
       st9(S.tmp_Ft + 9 * (size_t)p, Ft);
       st9(S.tmp_U + 9 * (size_t)p, U);
       st9(S.tmp_V + 9 * (size_t)p, V);
       st3(S.tmp_sig + 3 * (size_t)p, sig);
-    } else {
-      st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     }
+    if (MODE != kRecompute) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     key = pack_base(q.st.base);
   } else {
     q.st.base[0] = q.st.base[1] = q.st.base[2] = 0;
@@ -444,14 +448,20 @@ __device__ __forceinline__ NodeParams scene_node_params(const ddm_scene &S, cons
 
 // ---------------------------------------------------------------- grid update
 // ZERO=true (forward substep): consumes (m, mv) and clears them and the flags.
-template <bool ZERO>
+// SAVE=true (taped forward): also copies each listed block's (m, mv) and v, and the list,
+// into the tape (entries beyond its capacity are dropped; status[0] keeps the count).
+// need (optional): run only if *need != 0.
+template <bool ZERO, bool SAVE>
 __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                              const ddm_state *__restrict__ next_tab) {
+                                              const ddm_state *__restrict__ next_tab, ddm_grid_tape tape,
+                                              const int32_t *need) {
+  if (need && *need == 0) return;  // uniform over the grid
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
   const int n_listed = *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  if (SAVE && blockIdx.x == 0 && threadIdx.x == 0) tape.status[0] = n_listed;
   for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
     const int2 entry = list[e];
     const int blk = entry.x, si = entry.y;
@@ -465,10 +475,38 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
       v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P);
     }
     gv[gi] = make_float4(v.x, v.y, v.z, 0.f);
+    if (SAVE && e < tape.capacity) {
+      const size_t ti = ((size_t)e << 6) | (size_t)lane;
+      reinterpret_cast<float4 *>(tape.grid_mmv)[ti] = a;
+      reinterpret_cast<float4 *>(tape.grid_v)[ti] = make_float4(v.x, v.y, v.z, 0.f);
+      if (lane == 0) reinterpret_cast<int2 *>(tape.block_list)[e] = entry;
+    }
     if (ZERO) {
       mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
       if (lane == 0) B.block_flags[blk] = 0u;
     }
+  }
+}
+
+// Taped adjoint: puts a substep's grid back into the arena from its tape, or, when the
+// forward listed more blocks than the tape holds, leaves the (clear) arena alone and sets
+// status[1] so that the recompute kernels run.
+__global__ __launch_bounds__(kWG) void k_restore(ddm_batch B, ddm_grid_tape tape) {
+  const int n = tape.status[0];
+  const bool use = n <= tape.capacity;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    tape.status[1] = use ? 0 : 1;
+    if (use) *B.block_count = n;
+  }
+  if (!use) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int2 *saved = reinterpret_cast<const int2 *>(tape.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n; e += gridDim.x * (kWG / 64)) {
+    const int2 entry = saved[e];
+    const size_t gi = ((size_t)entry.x << 6) | (size_t)lane, ti = ((size_t)e << 6) | (size_t)lane;
+    reinterpret_cast<float4 *>(B.grid_mmv)[gi] = reinterpret_cast<const float4 *>(tape.grid_mmv)[ti];
+    reinterpret_cast<float4 *>(B.grid_v)[gi] = reinterpret_cast<const float4 *>(tape.grid_v)[ti];
+    if (lane == 0) reinterpret_cast<int2 *>(B.block_list)[e] = entry;
   }
 }
 
@@ -749,9 +787,9 @@ static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_st
   if (e) return e;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<false><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, nullptr);
   mark(ev, 1, stream);
-  k_grid<true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
+  k_grid<true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, ddm_grid_tape{}, nullptr);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
@@ -765,9 +803,9 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
   if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<true><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, nullptr);
   mark(ev, 1, stream);
-  k_grid<false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
+  k_grid<false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, ddm_grid_tape{}, nullptr);
   mark(ev, 2, stream);
   k_g2p_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, cur_grad, next_grad);
   mark(ev, 3, stream);
@@ -776,6 +814,74 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
   k_p2g_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, cur_grad, next_grad);
   mark(ev, 5, stream);
   return (int)hipGetLastError();
+}
+
+static inline int check_tape(const ddm_grid_tape *t) {
+  if (!t || !t->grid_mmv || !t->grid_v || !t->block_list || !t->status || t->capacity < 0) return DDM_EINVAL;
+  return 0;
+}
+
+static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *ev) {
+  int e = check(b, cur, next);
+  if (!e) e = check_tape(tape);
+  if (e) return e;
+  if (b->n_chunks == 0) return 0;
+  mark(ev, 0, stream);
+  k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, nullptr);
+  mark(ev, 1, stream);
+  k_grid<true, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, *tape, nullptr);
+  mark(ev, 2, stream);
+  k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  mark(ev, 3, stream);
+  return (int)hipGetLastError();
+}
+
+static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                 const ddm_state *cur_grad, const ddm_state *next_grad, const ddm_grid_tape *tape,
+                                 hipStream_t stream, hipEvent_t *ev) {
+  int e = check(b, cur, next);
+  if (!e) e = check_tape(tape);
+  if (e) return e;
+  if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
+  if (b->n_chunks == 0) return 0;
+  const int32_t *need = tape->status + 1;
+  mark(ev, 0, stream);
+  k_restore<<<grid_wgs(b), kWG, 0, stream>>>(*b, *tape);
+  k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, need);
+  k_grid<false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, ddm_grid_tape{}, need);
+  mark(ev, 1, stream);
+  k_g2p_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, cur_grad, next_grad);
+  mark(ev, 2, stream);
+  k_grid_adj<<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, cur_grad, next_grad);
+  mark(ev, 3, stream);
+  k_p2g_adj<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, cur_grad, next_grad);
+  mark(ev, 4, stream);
+  return (int)hipGetLastError();
+}
+
+DDM_API int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                      const ddm_grid_tape *tape, hipStream_t stream) {
+  return launch_forward_taped(b, cur, next, tape, stream, nullptr);
+}
+
+DDM_API int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                       const ddm_state *cur_grad, const ddm_state *next_grad,
+                                       const ddm_grid_tape *tape, hipStream_t stream) {
+  return launch_backward_taped(b, cur, next, cur_grad, next_grad, tape, stream, nullptr);
+}
+
+DDM_API int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                            const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events) {
+  if (!events) return DDM_EINVAL;
+  return launch_forward_taped(b, cur, next, tape, stream, events);
+}
+
+DDM_API int ddm_substep_backward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                             const ddm_state *cur_grad, const ddm_state *next_grad,
+                                             const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events) {
+  if (!events) return DDM_EINVAL;
+  return launch_backward_taped(b, cur, next, cur_grad, next_grad, tape, stream, events);
 }
 
 DDM_API int ddm_substep_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream) {

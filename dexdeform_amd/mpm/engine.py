@@ -291,6 +291,46 @@ class Engine:
                                                  gcur_tab.data_ptr(), gnxt_tab.data_ptr(), _lib.stream_ptr(stream)),
                    "ddm_substep_backward")
 
+    # ------------------------------------------------------------------ taped adjoint
+    def temps_table(self, temps: torch.Tensor) -> torch.Tensor:
+        """Scene table whose F_tmp/U/V/sigma temporaries live in ``temps`` (a float32
+        tensor of 30 * n_particles_total elements: F_tmp | U | V | sigma), so that a taped
+        substep keeps its own copy for its adjoint."""
+        P = self.n_particles_total
+        if temps.numel() < 30 * P or temps.dtype != torch.float32 or not temps.is_contiguous():
+            raise ValueError("temps must be a contiguous float32 tensor of 30 * n_particles_total elements")
+        base, f4 = int(temps.data_ptr()), 4
+        tab = self._scene_np.copy()
+        for i in range(len(tab)):
+            pb = int(self.p_begin[i])
+            tab[i]["tmp_Ft"] = base + 9 * pb * f4
+            tab[i]["tmp_U"] = base + (9 * P + 9 * pb) * f4
+            tab[i]["tmp_V"] = base + (18 * P + 9 * pb) * f4
+            tab[i]["tmp_sig"] = base + (27 * P + 3 * pb) * f4
+        return torch.as_tensor(tab.view(np.uint8).copy(), device=self.device)
+
+    def _batch_with_scenes(self, scene_tab):
+        b = _lib.DdmBatch.from_buffer_copy(self._batch)
+        b.scenes = scene_tab.data_ptr()
+        return b
+
+    def forward_taped(self, cur_tab, nxt_tab, scene_tab, tape, stream=None):
+        """Forward substep recording ``tape`` (a DdmGridTape) and, through ``scene_tab``
+        (from temps_table), the temporaries."""
+        b = self._batch_with_scenes(scene_tab)
+        _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+                                                      ctypes.byref(tape), _lib.stream_ptr(stream)),
+                   "ddm_substep_forward_taped")
+
+    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, tape, stream=None):
+        """Adjoint substep from the tape recorded by forward_taped on the same tables."""
+        if not self.with_grad:
+            raise RuntimeError("engine was created with with_grad=False")
+        b = self._batch_with_scenes(scene_tab)
+        _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+                                                       gcur_tab.data_ptr(), gnxt_tab.data_ptr(), ctypes.byref(tape),
+                                                       _lib.stream_ptr(stream)), "ddm_substep_backward_taped")
+
     def reset_grid(self, stream=None):
         _lib.check(self.lib.ddm_batch_reset(ctypes.byref(self._batch), _lib.stream_ptr(stream)), "ddm_batch_reset")
 

@@ -53,8 +53,39 @@ class _Particles:
         self.flat.zero_()
 
 
+class _Tapes:
+    """Per-substep tapes of one rollout segment: the SVD temporaries (30 floats per
+    particle, through a scene table of their own) and a snapshot of the listed grid blocks
+    (up to ``capacity`` blocks; beyond it the adjoint recomputes that substep)."""
+
+    def __init__(self, engine: Engine, slots: int, capacity: int):
+        dev = engine.device
+        P = engine.n_particles_total
+        self.capacity = int(capacity)
+        self.temps = torch.empty(slots, 30 * P, dtype=torch.float32, device=dev)
+        self.scene_tabs = [engine.temps_table(self.temps[i]) for i in range(slots)]
+        self.grid_mmv = torch.empty(slots, self.capacity * 64 * 4, dtype=torch.float32, device=dev)
+        self.grid_v = torch.empty_like(self.grid_mmv)
+        self.block_list = torch.empty(slots, self.capacity * 2, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(slots, 2, dtype=torch.int32, device=dev)
+        self.tapes = [_lib.DdmGridTape(self.grid_mmv[i].data_ptr(), self.grid_v[i].data_ptr(),
+                                       self.block_list[i].data_ptr(), self.status[i].data_ptr(), self.capacity, 0)
+                      for i in range(slots)]
+
+    def nbytes(self) -> int:
+        return sum(t.numel() * t.element_size() for t in (self.temps, self.grid_mmv, self.grid_v, self.block_list,
+                                                          self.status))
+
+
 class Rollout:
-    def __init__(self, engine: Engine, T: int, checkpoint_every: Optional[int] = None):
+    def __init__(self, engine: Engine, T: int, checkpoint_every: Optional[int] = None, tape: bool = False,
+                 tape_blocks: Optional[int] = None):
+        """tape=True keeps, for every substep of a segment, the SVD temporaries and the
+        grid of the forward pass, so the adjoint substeps skip recomputing P2G and the
+        grid update (120 B per particle plus 2 KiB per active grid block per substep).
+        tape_blocks bounds the blocks one substep's snapshot holds (default: all blocks
+        of the engine, capped at n_particles / 32 + 1024); a substep that lists more
+        is recomputed by its adjoint instead (decided on the device)."""
         self.engine = engine
         self.T = int(T)
         dev = engine.device
@@ -87,6 +118,10 @@ class Rollout:
             return engine.table(s.x, s.v, s.F, s.C, gp[t], gr[t])
 
         self._sview, self._gview = sview, gview
+        self.tape = None
+        if tape:
+            cap = tape_blocks or min(engine.n_blocks, engine.n_particles_total // 32 + 1024)
+            self.tape = _Tapes(engine, self.K, cap)
 
     # ------------------------------------------------------------------ inputs / outputs
     def set_initial(self, x, v=None, F=None, C=None):
@@ -111,19 +146,24 @@ class Rollout:
         return self.seg[self.K] if self.full else self._final
 
     # ------------------------------------------------------------------ passes
-    def _fwd(self, j, t):
-        self.engine.forward_tables(self._sview(j, t), self._sview(j + 1, t + 1))
+    def _fwd(self, j, t, taped=False):
+        if taped:
+            self.engine.forward_taped(self._sview(j, t), self._sview(j + 1, t + 1), self.tape.scene_tabs[j],
+                                      self.tape.tapes[j])
+        else:
+            self.engine.forward_tables(self._sview(j, t), self._sview(j + 1, t + 1))
 
     def forward(self):
         """T forward substeps from the initial state."""
         if self.full:
             for t in range(self.T):
-                self._fwd(t, t)
+                self._fwd(t, t, taped=self.tape is not None)
             return self.seg[self.T]
-        # checkpointed: run each segment in the segment buffer and keep its end state
+        # checkpointed: run each segment in the segment buffer and keep its end state;
+        # segment 0 restarts from checkpoint 0 too, since a previous pass left a later
+        # segment's start in the buffer
         for c in range(self.T // self.K):
-            if c > 0:
-                self.seg[0].copy_(self.ckpt[c])
+            self.seg[0].copy_(self.ckpt[c])
             for j in range(self.K):
                 self._fwd(j, c * self.K + j)
             self.ckpt[c + 1].copy_(self.seg[self.K])
@@ -147,14 +187,18 @@ class Rollout:
             if not self.full:
                 self.seg[0].copy_(self.ckpt[c])
                 for j in range(self.K):
-                    self._fwd(j, base + j)
+                    self._fwd(j, base + j, taped=self.tape is not None)
             for j in range(self.K - 1, -1, -1):
                 t = base + j
                 nxt_slot = cur_slot
                 cur_slot = 1 - cur_slot
                 self.g[cur_slot].zero_()
-                eng.backward_tables(self._sview(j, t), self._sview(j + 1, t + 1), self._gview(cur_slot, t),
-                                    self._gview(nxt_slot, t + 1))
+                tabs = (self._sview(j, t), self._sview(j + 1, t + 1), self._gview(cur_slot, t),
+                        self._gview(nxt_slot, t + 1))
+                if self.tape is not None:
+                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], self.tape.tapes[j])
+                else:
+                    eng.backward_tables(*tabs)
         self._g0 = self.g[cur_slot]
         return self._g0
 
@@ -190,4 +234,8 @@ class Rollout:
 
     def state_bytes(self) -> int:
         per = self.engine.n_particles_total * (3 + 3 + 9 + 9) * 4
-        return per * (len(self.ckpt) + len(self.seg) + len(self.g))
+        return per * (len(self.ckpt) + len(self.seg) + len(self.g)) + (self.tape.nbytes() if self.tape else 0)
+
+    def tape_recomputed(self) -> int:
+        """Substeps of the last backward segment that had to recompute (snapshot full)."""
+        return int(self.tape.status[:, 1].sum().item()) if self.tape is not None else 0

@@ -213,8 +213,8 @@ def hand_actions(n_env_steps: int, n_joints=22, seed=4) -> np.ndarray:
 def large_scene(n_particles=1_000_000, seed=6, grid=128, dt=5e-5, friction=0.9):
     """Large single scene: a 128^3 grid, particles filling a union of 3 seeded boxes
     (half-extents U[0.06, 0.14] x U[0.05, 0.10] x U[0.06, 0.14], resting near the floor,
-    centres within 0.15 of (0.5, 0.5)) under the 22-link capsule hand. Returns (scene,
-    hinge tree, initial joint angles)."""
+    centres within 0.15 of (0.5, 0.5)) with the 22-link capsule hand (palm at height 0.30)
+    reaching into them. Returns (scene, hinge tree, initial joint angles)."""
     from .kinematics import synthetic_hand
     rng = np.random.Generator(np.random.PCG64(seed))
     boxes = []
@@ -224,7 +224,7 @@ def large_scene(n_particles=1_000_000, seed=6, grid=128, dt=5e-5, friction=0.9):
         boxes.append((c - half, c + half))
     dx = 1.0 / grid
     x0 = union_boxes_particles(n_particles, boxes, int(rng.integers(1 << 30)), dx)
-    tree, args, q0 = synthetic_hand(seed=4)
+    tree, args, q0 = synthetic_hand(seed=4, palm=(0.5, 0.30, 0.5))
     pos, rot = tree.fk(q0)
     sc = make_scene(x0, grid, dx, dt, capsule_prims(len(args), friction), args, pos.numpy(), rot.numpy())
     sc.meta["boxes"] = boxes

@@ -172,6 +172,46 @@ int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur, const d
                                const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
                                hipEvent_t *events);
 
+/* Grid snapshot of one substep for the taped adjoint. No counterpart in the reference:
+ * its backward always recomputes compute_svd, p2g and grid_op_v2 (simulator.py:573-585).
+ * For callers with the memory to keep one snapshot (plus the F_tmp/U/V/sigma temporaries,
+ * 120 B per particle) per substep of a rollout segment, so that the adjoint substep skips
+ * that recompute. Device pointers; the struct itself is passed from the host. */
+typedef struct ddm_grid_tape {
+  float *grid_mmv;             /* [capacity][64][4] (m, mv) of the listed blocks, list order */
+  float *grid_v;               /* [capacity][64][4] node velocity after the grid update */
+  int32_t *block_list;         /* [capacity][2] (block, scene index) */
+  int32_t *status;             /* [2]: [0] blocks the substep listed (written by the forward);
+                                  [1] 1 if the backward had to recompute (more blocks than
+                                  capacity), else 0 (written by the backward) */
+  int32_t capacity;            /* blocks the snapshot holds */
+  int32_t reserved;
+} ddm_grid_tape;               /* 40 bytes */
+
+/* Forward substep that also records the tape: the temporaries go to the scenes'
+ * tmp_Ft/tmp_U/tmp_V/tmp_sig (give each taped substep its own scene table) and the grid
+ * of the listed blocks to *tape. Same results as ddm_substep_forward. */
+int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                              const ddm_grid_tape *tape, hipStream_t stream);
+
+/* Adjoint substep from a tape recorded by ddm_substep_forward_taped on the same states,
+ * poses and scene table: restores the grid instead of recomputing P2G and the grid update.
+ * When the forward listed more blocks than the tape holds, the kernels recompute them
+ * (decided on the device, no host synchronisation) and set tape->status[1]. Same results
+ * as ddm_substep_backward up to FP32 accumulation order. */
+int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                               const ddm_state *cur_grad, const ddm_state *next_grad,
+                               const ddm_grid_tape *tape, hipStream_t stream);
+
+/* The taped substeps with hipEvents recorded between their stages (as the _timed
+ * variants above): forward events[0..3] around P2G, grid update, G2P; backward events[0..4]
+ * around restore (+ recompute when needed), G2P adjoint, grid adjoint, P2G adjoint. */
+int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                    const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events);
+int ddm_substep_backward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                     const ddm_state *cur_grad, const ddm_state *next_grad,
+                                     const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events);
+
 /* Zeroes the batch's grid arenas and block flags (after an error or on setup). */
 int ddm_batch_reset(const ddm_batch *b, hipStream_t stream);
 

@@ -46,8 +46,12 @@ def main():
             out[name] = pick(H.oracle_action_grad(sc, tree, q0, acts, target, dt))
             print(f"{name} adjoint done in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     t0 = time.time()
-    fd = H.fd_action_grad(sc, tree, q0, acts, target, coords)
-    print(f"finite differences done in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    fd = []
+    for i, c in enumerate(coords):  # one coordinate at a time, with a progress line each
+        fd.append(H.fd_action_grad(sc, tree, q0, acts, target, [c])[0])
+        print(f"finite difference {i + 1}/{len(coords)} {c}: {fd[-1]:+.6e} ({time.time() - t0:.0f} s)",
+              file=sys.stderr, flush=True)
+    fd = np.array(fd)
     res = {"gpu_vs_fd": rel_err(pick(g_gpu), fd)}
     for name in ("oracle_fp32", "oracle_fp64"):
         if name in out:

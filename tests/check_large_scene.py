@@ -9,8 +9,9 @@ coordinates as one JSON line.
 
 Gradient check: the FP64 oracle cannot run 16 rollouts of 500 substeps over 10^6 particles
 in reasonable time, so the check is against FP32 central differences of the GPU rollout
-itself (eps = 1e-2 in action units). Their rounding noise (FP32 state, atomic order) is
-reported beside the error: the same kernels are checked against FP64 finite differences
+itself (eps = 0.1 in action units, i.e. joint-angle changes below 7e-4 rad), per
+coordinate and along the gradient direction. Their rounding noise (FP32 state, atomic
+order) is reported beside the error: the same kernels are checked against FP64 finite differences
 at the smaller hand-contact size by tests/test_gpu_hand.py.
 """
 import argparse
@@ -38,7 +39,7 @@ def main():
     ap.add_argument("--env-steps", type=int, default=50)
     ap.add_argument("--checkpoint", type=int, default=25)
     ap.add_argument("--coords", type=int, default=8)
-    ap.add_argument("--eps", type=float, default=1e-2)
+    ap.add_argument("--eps", type=float, default=0.1)
     a = ap.parse_args()
     dev = torch.device("cuda:0")
     sc, tree, q0 = W.large_scene(a.particles)
@@ -81,11 +82,18 @@ def main():
         fd.append((lp - lm) / (2 * a.eps))
     l0 = [loss_of(acts) for _ in range(3)]  # run-to-run spread of the loss (atomic order)
     gpick = np.array([g[e, k] for e, k in coords])
+    # directional derivative along the gradient: the largest loss change per unit step,
+    # so the best signal-to-noise ratio a finite difference of this rollout can have
+    gn = np.linalg.norm(g)
+    dirv = (g / gn).astype(np.float32)
+    dd = (loss_of(acts + a.eps * dirv) - loss_of(acts - a.eps * dirv)) / (2 * a.eps)
     out = {"particles": sc.n, "grid": 128, "substeps": T, "checkpoint_every": a.checkpoint, "bodies": sc.nb,
            "loss": loss, "fwd_bwd_seconds": el, "particle_substeps_per_s": sc.n * T / el,
            "peak_device_bytes": int(torch.cuda.max_memory_allocated()),
            "state_bytes": ro.state_bytes(), "coords": coords, "grad": gpick.tolist(), "fd_fp32": fd,
            "grad_vs_fd_fp32": rel_err(gpick, np.array(fd)),
+           "grad_norm": float(gn), "directional_fd_fp32": float(dd),
+           "directional_rel_err": float(abs(dd - gn) / gn),
            "loss_run_to_run_spread": float(np.ptp(l0)), "fd_noise_estimate": float(np.ptp(l0) / (2 * a.eps))}
     print(json.dumps(out), flush=True)
 

@@ -43,10 +43,10 @@ def _poses(sc, T, seed):
     return (sc.body_pos[None] + t * vel[None]).astype(np.float32), np.repeat(sc.body_rot[None], T + 1, 0)
 
 
-def _rollout(scenes, T, K=None, pose_seeds=None, grad_scale=None):
+def _rollout(scenes, T, K=None, pose_seeds=None, grad_scale=None, tape=False, tape_blocks=None):
     dev = _dev()
     eng = Engine([s.spec for s in scenes], device=dev)
-    ro = Rollout(eng, T, checkpoint_every=K)
+    ro = Rollout(eng, T, checkpoint_every=K, tape=tape, tape_blocks=tape_blocks)
     ro.set_initial(torch.tensor(np.concatenate([s.x0 for s in scenes]), device=dev))
     seeds = pose_seeds or [100 + i for i in range(len(scenes))]
     P = [_poses(s, T, sd) for s, sd in zip(scenes, seeds)]
@@ -209,3 +209,40 @@ def test_captured_rollout_graph_replays_match_eager():
         other.backward(x_grad=seed)
         _, opg, _ = outputs(other)
         assert rel_err(opg, scale * ref[1]) <= 1e-4
+
+
+def test_repeated_checkpointed_forward_passes_restart_from_the_initial_state():
+    """Forward-only passes of a checkpointed rollout (as finite differences make them)
+    each start from the initial state: the second pass reproduces the first."""
+    dev = _dev()
+    sc = _scene(3000, 48, seed=5)
+    eng = Engine([sc.spec], device=dev)
+    ro = Rollout(eng, 20, checkpoint_every=5)
+    ro.set_initial(torch.tensor(sc.x0, device=dev))
+    pos, rot = _poses(sc, 20, 7)
+    ro.set_poses(torch.tensor(pos, device=dev), torch.tensor(rot, device=dev))
+    a = ro.forward().x.clone()
+    b = ro.forward().x.clone()
+    torch.cuda.synchronize()
+    e = rel_err(b.cpu().numpy(), a.cpu().numpy())
+    print(f"second forward pass vs first: x {e:.2e}")
+    assert e <= 1e-6
+
+
+@pytest.mark.parametrize("K", [None, 10])
+def test_taped_adjoint_matches_recompute(K):
+    """The taped adjoint (grid snapshot + temporaries of the forward pass) gives the
+    recomputing adjoint's gradients; with a snapshot too small for the substeps' blocks,
+    every adjoint substep falls back to recomputing, with the same result."""
+    scenes = [_scene(3000, 48, seed=2), _scene(6000, 64, seed=3, nb=5)]
+    _, _, x_ref, pg_ref, g0_ref = _rollout(scenes, 20, K=K)
+    _, ro, x_tp, pg_tp, g0_tp = _rollout(scenes, 20, K=K, tape=True)
+    assert ro.tape_recomputed() == 0
+    e = [rel_err(x_tp, x_ref), rel_err(pg_tp, pg_ref), rel_err(g0_tp, g0_ref)]
+    print(f"taped vs recompute (K={K}): x {e[0]:.2e} pose grad {e[1]:.2e} x0 grad {e[2]:.2e}")
+    assert e[0] <= 1e-6 and e[1] <= 1e-4 and e[2] <= 1e-4
+    _, ro, x_of, pg_of, g0_of = _rollout(scenes, 20, K=K, tape=True, tape_blocks=4)
+    assert ro.tape_recomputed() == ro.K, "a 4-block snapshot cannot hold these substeps"
+    e = [rel_err(pg_of, pg_ref), rel_err(g0_of, g0_ref)]
+    print(f"overflowing tape (K={K}): pose grad {e[0]:.2e} x0 grad {e[1]:.2e}")
+    assert e[0] <= 1e-4 and e[1] <= 1e-4
