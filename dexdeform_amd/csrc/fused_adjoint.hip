@@ -1,0 +1,237 @@
+// Adjoint kernels of the fused multi-scene substep. Compiled with FMA contraction
+// (see fused_common.h); the recompute kernels of the adjoint pass stay in
+// fused_kernels.hip with the forward kernels.
+#include "fused_common.h"
+
+namespace ddm_fused {
+namespace {
+
+// adjoint of the grid update; clears (m, mv), the velocity adjoint and the flags
+__global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                  const ddm_state *__restrict__ next_tab,
+                                                  const ddm_state *__restrict__ cur_grad_tab,
+                                                  const ddm_state *__restrict__ next_grad_tab) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
+  float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
+  float4 *dmmv = reinterpret_cast<float4 *>(B.grid_dmmv);
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const int2 entry = list[e];
+    const int blk = entry.x, si = entry.y;
+    const ddm_scene &S = B.scenes[si];
+    BlockCoord c = block_node(S, blk, lane);
+    size_t gi = ((size_t)blk << 6) | (size_t)lane;
+    float4 a = mmv[gi];
+    float4 d = dv[gi];
+    const float m = a.x;
+    const f3 mv{a.y, a.z, a.w};
+    const bool act = node_active(m);
+    NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
+    const f3 gx = node_position(c.ix, c.iy, c.iz, P.dx);
+    const int mode = S.adjoint_mode;
+    // forward replay keeping the velocities entering each contact (integrator.cu:678-732)
+    f3 tape_v[kTapeMax];
+    int tape_b[kTapeMax];
+    int nt = 0;
+    f3 v0{0.f, 0.f, 0.f}, g{0.f, 0.f, 0.f};
+    if (act) {
+      f3 v = mv * (1.f / m);
+      v += P.dt * P.gravity;
+      v0 = v;
+      for (int b = 0; b < P.nb; ++b) {
+        f3 vin = v;
+        if (contact(v, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b), P.dt) && nt < kTapeMax) {
+          tape_v[nt] = vin;
+          tape_b[nt] = b;
+          ++nt;
+        }
+      }
+      g = boundary_adj(v, f3{d.x, d.y, d.z}, c.ix, c.iy, c.iz, P, mode);
+    }
+    // bodies in reverse order, uniformly across the wave so pose gradients can be reduced
+    for (int b = P.nb - 1; b >= 0; --b) {
+      PoseGrad pg = posegrad_zero();
+      bool mine = false;
+      if (act) {
+        Body Bd = load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b);
+        float infl;
+        if (in_contact(prim_sdf(Bd.prim, Bd.args, to_body(Bd.pos, Bd.rot, gx)), Bd.prim[2], infl)) {
+          mine = true;
+          f3 vin;
+          if (nt > 0 && tape_b[nt - 1] == b) {
+            vin = tape_v[--nt];
+          } else {  // tape overflow: replay the contacts before b
+            vin = v0;
+            for (int b2 = 0; b2 < b; ++b2)
+              contact(vin, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b2), P.dt);
+          }
+          contact_adj(vin, gx, Bd, P.dt, mode, g, pg);
+        }
+      }
+      if (__any(mine)) {
+        float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
+                          pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
+#pragma unroll
+        for (int e = 0; e < 14; ++e) vals[e] = wave_sum(vals[e]);
+        if (lane == 0) {
+          const ddm_state &cg = cur_grad_tab[si];
+          const ddm_state &ng = next_grad_tab[si];
+          float *cp = cg.body_pos + 3 * b, *cr = cg.body_rot + 4 * b;
+          float *np = ng.body_pos + 3 * b, *nr = ng.body_rot + 4 * b;
+          atomicAdd(cp + 0, vals[0]);
+          atomicAdd(cp + 1, vals[1]);
+          atomicAdd(cp + 2, vals[2]);
+          atomicAdd(cr + 0, vals[3]);
+          atomicAdd(cr + 1, vals[4]);
+          atomicAdd(cr + 2, vals[5]);
+          atomicAdd(cr + 3, vals[6]);
+          atomicAdd(np + 0, vals[7]);
+          atomicAdd(np + 1, vals[8]);
+          atomicAdd(np + 2, vals[9]);
+          atomicAdd(nr + 0, vals[10]);
+          atomicAdd(nr + 1, vals[11]);
+          atomicAdd(nr + 2, vals[12]);
+          atomicAdd(nr + 3, vals[13]);
+        }
+      }
+    }
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+      f3 gmv = (float)(1.0 / (double)m) * g;  // integrator.cu:1054-1055
+      out = make_float4((-1.f / m / m) * dot3(mv, g), gmv.x, gmv.y, gmv.z);
+    }
+    dmmv[gi] = out;
+    mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+    dv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane == 0) B.block_flags[blk] = 0u;
+  }
+}
+
+// adjoint of G2P: adds to the x adjoint and scatters the node-velocity adjoint
+__global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                 const ddm_state *__restrict__ next_tab,
+                                                 const ddm_state *__restrict__ cur_grad_tab,
+                                                 const ddm_state *__restrict__ next_grad_tab) {
+  __shared__ ScatterLDS<3, kTileG2P> sh;
+  __shared__ StencilRecs rc;
+  __shared__ Groups gr;
+  __shared__ int s_box[6];
+  float *const tx = sh.tile[0], *const ty = sh.tile[1], *const tz = sh.tile[2];
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
+  const ddm_state cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  const bool valid = p < ch.z;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  Stencil st;
+  f3 g_nv{0.f, 0.f, 0.f};
+  m33 gC;
+  int key = 0;
+  if (valid) {
+    BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
+    NullScatter none;
+    gC = ld9(next_grad.C + 9 * (size_t)p);
+    f3 gxp = g2p_particle_adj(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
+                              ld3(next_grad.x + 3 * (size_t)p), ld3(next_grad.v + 3 * (size_t)p), gC, gv, none,
+                              S.inv_dx, S.dx, S.dt, S.ground_height, dim, &g_nv, &st);
+    add3(cur_grad.x + 3 * (size_t)p, gxp);
+    key = pack_base(st.base);
+  } else {
+    st.base[0] = st.base[1] = st.base[2] = 0;
+  }
+  const Tile T = chunk_tile(valid, st.base, s_box);
+  int pos;
+  const int G = build_groups(valid, key, min(kWG, ch.z - ch.y), sh.grp, gr, pos);
+  if (valid) stage_record(rc, pos, st, 0.f, g_nv, gC);
+  const float inv_dx = S.inv_dx;
+  auto contrib = [inv_dx](const StencilRecs &r, int s, int i, int j, int k, float w, float *acc) {
+    float4 a = r.f0[s];
+    f3 g = g2p_adj_contrib(w, f3{a.x, a.y, a.z}, rec_vec(r, s), rec_mat(r, s), inv_dx, i, j, k);
+    acc[0] += g.x;
+    acc[1] += g.y;
+    acc[2] += g.z;
+  };
+  if (slabs_fit<kTileG2P>(T)) {
+    const int sx = slab_width<kTileG2P>(T);
+    for (int xs = T.lo[0]; xs < T.lo[0] + T.t[0]; xs += sx) {  // uniform over the workgroup
+      const Tile L = slab(T, xs, sx);
+      if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
+      for (int i = threadIdx.x; i < L.n; i += kWG) tx[i] = ty[i] = tz[i] = 0.f;
+      __syncthreads();  // also publishes the staged records
+      group_tasks<3>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
+        const int li = L.local(X, Y, Z);
+        atomicAdd(&tx[li], a[0]);
+        atomicAdd(&ty[li], a[1]);
+        atomicAdd(&tz[li], a[2]);
+      });
+      __syncthreads();
+      for (int i = threadIdx.x; i < L.n; i += kWG) {
+        float a = tx[i], b = ty[i], c = tz[i];
+        if (a != 0.f || b != 0.f || c != 0.f) {
+          int xx = L.lo[0] + i / (L.t[1] * L.t[2]), yy = L.lo[1] + (i / L.t[2]) % L.t[1], zz = L.lo[2] + i % L.t[2];
+          float *g = B.grid_dv + 4 * (size_t)node_index(S, xx, yy, zz);
+          atomicAdd(g + 0, a);
+          atomicAdd(g + 1, b);
+          atomicAdd(g + 2, c);
+        }
+      }
+    }
+  } else {
+    __syncthreads();  // publishes the staged records
+    group_tasks<3>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
+      float *g = B.grid_dv + 4 * (size_t)node_index(S, X, Y, Z);
+      atomicAdd(g + 0, a[0]);
+      atomicAdd(g + 1, a[1]);
+      atomicAdd(g + 2, a[2]);
+    });
+  }
+}
+
+// adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
+__global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                 const ddm_state *__restrict__ cur_grad_tab,
+                                                 const ddm_state *__restrict__ next_grad_tab) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  if (p >= ch.z) return;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  const size_t p3 = 3 * (size_t)p, p9 = 9 * (size_t)p;
+  BlockedGatherDMMV gat{reinterpret_cast<const float4 *>(B.grid_dmmv), &S};
+  m33 F = ld9(cur.F + p9), C = ld9(cur.C + p9);
+  m33 Ft = ld9(S.tmp_Ft + p9), U = ld9(S.tmp_U + p9), V = ld9(S.tmp_V + p9);
+  f3 sig = ld3(S.tmp_sig + p3);
+  P2GAdj a;
+  p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+                   ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
+  m33 gFt, gF, gC;
+  svd_adj(F, C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
+  add3(cur_grad.x + p3, a.x);
+  add3(cur_grad.v + p3, a.v);
+  m33 cg = ld9(cur_grad.C + p9);
+  cg = madd(madd(cg, a.C), gC);  // p2g_grad adds first, then compute_svd_grad (simulator.py:584-585)
+  st9(cur_grad.C + p9, cg);
+  add9(cur_grad.F + p9, gF);
+}
+
+}  // namespace
+
+void launch_g2p_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
+                    const ddm_state *next_grad, hipStream_t stream) {
+  k_g2p_adj<<<b.n_chunks, kWG, 0, stream>>>(b, cur, next, cur_grad, next_grad);
+}
+void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
+                     const ddm_state *next_grad, hipStream_t stream) {
+  k_grid_adj<<<grid_wgs(&b), kWG, 0, stream>>>(b, cur, next, cur_grad, next_grad);
+}
+void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
+                    hipStream_t stream) {
+  k_p2g_adj<<<b.n_chunks, kWG, 0, stream>>>(b, cur, cur_grad, next_grad);
+}
+
+}  // namespace ddm_fused

@@ -1,0 +1,375 @@
+// Shared device code of the fused multi-scene substep (forward kernels in
+// fused_kernels.hip, adjoint kernels in fused_adjoint.hip).
+//
+// Layout and invariants
+//  * Scenes are independent: per-scene records (ddm_scene, ddm_state) in device memory
+//    give each scene's particle arrays, material, primitives and poses; a chunk table
+//    maps each workgroup to <= 256 particles of one scene (local indices).
+//  * Each scene's grid is stored in 4x4x4 node blocks (64 nodes = one wavefront),
+//    blocks of all scenes concatenated; every grid array is float4 per node.
+//  * Block-sparse: P2G marks the blocks it touches in block_flags and appends each newly
+//    marked block, with its scene, to block_list; the grid kernels visit exactly the
+//    listed blocks. Between substeps grid_mmv, grid_dv, block_flags and block_count are
+//    all zero: the last grid kernel of a substep clears what it consumed and the kernel
+//    after it resets the count, so no dense memset is ever needed.
+//  * P2G and the G2P adjoint group the chunk's particles by stencil base (a hash table
+//    and a counting sort in LDS), stage each particle's stencil data in LDS in group
+//    order and scatter one (group, stencil node) task per thread: the task sums its
+//    group in registers and adds one value per component to an LDS tile covering the
+//    chunk's stencil box. LDS float atomics are the throughput limit of this scatter, so
+//    a group of n particles costs one atomic instead of n, whatever the particle order.
+//  * The tile is flushed to the grid with global atomics for its non-zero nodes. A box
+//    larger than the tile is processed in x-slabs, one tile pass per slab; only a box
+//    whose single x-plane exceeds the tile sends the tasks' sums to global atomics.
+//  * The contact adjoint reduces body-pose gradients across the wavefront before one
+//    atomic per body per wave.
+#pragma once
+#include <limits.h>
+#include "mpm_device.h"
+#include "../../include/dexdeform_mpm.h"
+
+namespace ddm_fused {
+
+using namespace ddm;
+
+namespace {
+
+constexpr int kWG = 256;
+// LDS tile capacity in nodes. P2G keeps 4 floats per node and the G2P adjoint 3; with
+// the staged particle records both stay under 40 KB of LDS per workgroup, so four
+// workgroups (one wave per SIMD each) fit a CU.
+constexpr int kTileP2G = 704;
+constexpr int kTileG2P = 960;
+constexpr int kHash = 512;       // open-addressing table of the chunk's stencil bases
+constexpr int kTapeMax = 8;      // contact tape entries kept per node in the adjoint
+
+__device__ __forceinline__ int node_index(const ddm_scene &S, int x, int y, int z) {
+  int nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
+  int blk = ((x >> 2) * nby + (y >> 2)) * nbz + (z >> 2);
+  return ((S.blk_begin + blk) << 6) | (((x & 3) << 4) | ((y & 3) << 2) | (z & 3));
+}
+__device__ __forceinline__ int block_of(const ddm_scene &S, int bx, int by, int bz) {
+  return S.blk_begin + (bx * (S.dim[1] >> 2) + by) * (S.dim[2] >> 2) + bz;
+}
+
+struct BlockedV {
+  const float4 *g;
+  const ddm_scene *S;
+  __device__ f3 operator()(int x, int y, int z) const {
+    float4 a = g[node_index(*S, x, y, z)];
+    return f3{a.x, a.y, a.z};
+  }
+};
+struct BlockedGatherDMMV {
+  const float4 *g;
+  const ddm_scene *S;
+  __device__ void operator()(int x, int y, int z, f3 &mv, float &m) const {
+    float4 a = g[node_index(*S, x, y, z)];
+    m = a.x;
+    mv = f3{a.y, a.z, a.w};
+  }
+};
+
+// ---------------------------------------------------------------- LDS tile helpers
+struct Tile {
+  int lo[3], t[3], n;
+  __device__ int local(int x, int y, int z) const { return ((x - lo[0]) * t[1] + (y - lo[1])) * t[2] + (z - lo[2]); }
+  __device__ bool has_x(int x) const { return x >= lo[0] && x < lo[0] + t[0]; }
+};
+// nodes per x-plane of the chunk box must fit the tile for the slab passes
+template <int CAP>
+__device__ __forceinline__ bool slabs_fit(const Tile &T) { return T.t[1] * T.t[2] <= CAP; }
+// x-planes per slab pass
+template <int CAP>
+__device__ __forceinline__ int slab_width(const Tile &T) { return min(T.t[0], CAP / (T.t[1] * T.t[2])); }
+// the slab of the chunk box starting at x-plane xs
+__device__ __forceinline__ Tile slab(const Tile &T, int xs, int sx) {
+  Tile L = T;
+  L.lo[0] = xs;
+  L.t[0] = min(sx, T.lo[0] + T.t[0] - xs);
+  L.n = L.t[0] * T.t[1] * T.t[2];
+  return L;
+}
+
+// ---------------------------------------------------------------- stencil groups
+// Particles of a chunk that share a stencil base form a group. A scatter task is one
+// (group, stencil node) pair: it sums the group's contributions to that node in
+// registers and adds the sum to the tile with one atomic per component, instead of
+// one atomic per particle.
+//
+// Per-particle stencil data staged in LDS, in group order. f1..f3 hold a vector and a
+// row-major 3x3 matrix whose meaning is the kernel's (P2G: m v and the affine momentum
+// matrix; G2P adjoint: node-velocity adjoint seed and C adjoint).
+struct StencilRecs {
+  float4 f0[kWG];    // fx (3), scalar
+  float4 f1[kWG];    // vector (3), matrix[0]
+  float4 f2[kWG];    // matrix[1..4]
+  float4 f3[kWG];    // matrix[5..8]
+  float w[kWG * 9];  // B-spline weights w[axis][offset], 9 consecutive per particle
+};
+__device__ __forceinline__ void stage_record(StencilRecs &rc, int s, const Stencil &st, float scalar, f3 vec,
+                                             const m33 &M) {
+  rc.f0[s] = make_float4(st.fx[0], st.fx[1], st.fx[2], scalar);
+  rc.f1[s] = make_float4(vec.x, vec.y, vec.z, M.a[0]);
+  rc.f2[s] = make_float4(M.a[1], M.a[2], M.a[3], M.a[4]);
+  rc.f3[s] = make_float4(M.a[5], M.a[6], M.a[7], M.a[8]);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) rc.w[9 * s + e] = st.w[e / 3][e % 3];
+}
+__device__ __forceinline__ f3 rec_vec(const StencilRecs &rc, int s) {
+  float4 b = rc.f1[s];
+  return f3{b.x, b.y, b.z};
+}
+__device__ __forceinline__ m33 rec_mat(const StencilRecs &rc, int s) {
+  float4 b = rc.f1[s], c = rc.f2[s], d = rc.f3[s];
+  return m33{{b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w}};
+}
+// stencil base packed into one non-negative int (grid dimensions are at most 1024)
+__device__ __forceinline__ int pack_base(const int b[3]) { return (b[0] << 20) | (b[1] << 10) | b[2]; }
+__device__ __forceinline__ void unpack_base(int key, int b[3]) {
+  b[0] = key >> 20;
+  b[1] = (key >> 10) & 1023;
+  b[2] = key & 1023;
+}
+
+// LDS used only while grouping; it shares storage with the scatter tile.
+struct GroupScratch {
+  int hkey[kHash];  // base key per table entry, -1 when empty
+  int hgrp[kHash];  // group index of the entry
+  int cnt[kWG];     // group sizes
+};
+// Group layout shared with the scatter tasks.
+struct Groups {
+  int off[kWG + 1];  // group g holds record slots off[g] .. off[g+1]-1
+  int key[kWG];      // packed stencil base of group g
+  int wcnt[kWG / 64];
+};
+
+// Groups the chunk's valid slots (a prefix of the chunk) by stencil base. Returns the
+// number of groups G (uniform) and, for a valid slot, the record slot pos where its
+// data goes: groups are contiguous and in the order their first member was inserted.
+// Ends with a barrier, after which the scratch is free.
+__device__ int build_groups(bool valid, int key, int n_valid, GroupScratch &sc, Groups &gr, int &pos) {
+  const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int i = t; i < kHash; i += kWG) sc.hkey[i] = -1;
+  sc.cnt[t] = 0;
+  __syncthreads();
+  int h = 0;
+  bool leader = false;
+  if (valid) {
+    // the low 3 bits of each coordinate: collision-free for bases spanning <= 8 cells per axis
+    h = ((key >> 14) & 0x1c0) | ((key >> 7) & 0x38) | (key & 7);
+    for (int probe = 0; probe < kHash; ++probe) {  // at most kWG keys: a free entry exists
+      const int old = atomicCAS(&sc.hkey[h], -1, key);
+      if (old == -1) {
+        leader = true;
+        break;
+      }
+      if (old == key) break;
+      h = (h + 1) & (kHash - 1);
+    }
+  }
+  unsigned long long m = __ballot(leader);
+  if (lane == 0) gr.wcnt[wave] = __popcll(m);
+  __syncthreads();
+  int G = 0, off = 0;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) {
+    const int c = gr.wcnt[w];
+    off += w < wave ? c : 0;
+    G += c;
+  }
+  if (leader) {
+    const int g = off + __popcll(m & ((1ull << lane) - 1ull));
+    sc.hgrp[h] = g;
+    gr.key[g] = key;
+  }
+  __syncthreads();
+  int g = 0, rank = 0;
+  if (valid) {
+    g = sc.hgrp[h];
+    rank = atomicAdd(&sc.cnt[g], 1);
+  }
+  __syncthreads();
+  // exclusive scan of the group sizes
+  const int c = t < G ? sc.cnt[t] : 0;
+  int v = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) gr.wcnt[wave] = v;
+  __syncthreads();
+  off = 0;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) off += w < wave ? gr.wcnt[w] : 0;
+  if (t < G) gr.off[t] = off + v - c;
+  if (t == 0) gr.off[G] = n_valid;
+  __syncthreads();
+  pos = valid ? gr.off[g] + rank : 0;
+  return G;
+}
+
+// Every (group, stencil node) task whose node lies in slab L (all tasks when L is null):
+// acc = sum over the group's records of contrib(rc, slot, i, j, k, w), then
+// sink(X, Y, Z, acc). Tasks are spread over the workgroup's threads.
+template <int NC, class Contrib, class Sink>
+__device__ void group_tasks(int G, const Groups &gr, const StencilRecs &rc, const Tile *L, Contrib contrib,
+                            Sink sink) {
+  for (int task = (int)threadIdx.x; task < 27 * G; task += kWG) {
+    const int g = task / 27, o = task - 27 * (task / 27);
+    int i, j, k;
+    offset_ijk(o, i, j, k);
+    int b[3];
+    unpack_base(gr.key[g], b);
+    if (L && !L->has_x(b[0] + i)) continue;
+    float acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = 0.f;
+    for (int q = gr.off[g], qe = gr.off[g + 1]; q < qe; ++q) {
+      const float w = rc.w[9 * q + i] * rc.w[9 * q + 3 + j] * rc.w[9 * q + 6 + k];  // weight_dyn's order
+      contrib(rc, q, i, j, k, w, acc);
+    }
+    sink(b[0] + i, b[1] + j, b[2] + k, acc);
+  }
+}
+
+// the scatter tile shares LDS with the grouping scratch, which is dead by then
+template <int NC, int CAP>
+union ScatterLDS {
+  float tile[NC][CAP];
+  GroupScratch grp;
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// bounding box of the chunk's stencil bases -> tile covering all stencil nodes.
+// Reduced across each wavefront first, then one LDS atomic per wave and bound.
+__device__ Tile chunk_tile(bool valid, const int base[3], int *s_box) {
+  if (threadIdx.x < 3) {
+    s_box[threadIdx.x] = INT_MAX;
+    s_box[3 + threadIdx.x] = INT_MIN;
+  }
+  __syncthreads();
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = wave_min(valid ? base[d] : INT_MAX);
+    hi[d] = wave_max(valid ? base[d] : INT_MIN);
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&s_box[d], lo[d]);
+      atomicMax(&s_box[3 + d], hi[d]);
+    }
+  }
+  __syncthreads();
+  Tile T;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    T.lo[d] = s_box[d];
+    T.t[d] = s_box[3 + d] - s_box[d] + 3;
+  }
+  T.n = T.t[0] * T.t[1] * T.t[2];
+  __syncthreads();
+  return T;
+}
+
+// mark a block of scene si; the first marker of a substep appends it to the block list
+__device__ __forceinline__ void mark_block(const ddm_batch &B, int blk, int si) {
+  if (atomicExch(&B.block_flags[blk], 1u) == 0u) {
+    const int e = atomicAdd(B.block_count, 1);
+    reinterpret_cast<int2 *>(B.block_list)[e] = make_int2(blk, si);
+  }
+}
+// mark every node block that intersects the tile
+__device__ void flag_tile_blocks(const ddm_batch &B, const ddm_scene &S, int si, const Tile &T) {
+  int b0[3], nb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b0[d] = T.lo[d] >> 2;
+    nb[d] = ((T.lo[d] + T.t[d] - 1) >> 2) - b0[d] + 1;
+  }
+  int tot = nb[0] * nb[1] * nb[2];
+  for (int i = threadIdx.x; i < tot; i += kWG) {
+    int bx = i / (nb[1] * nb[2]), by = (i / nb[2]) % nb[1], bz = i % nb[2];
+    mark_block(B, block_of(S, b0[0] + bx, b0[1] + by, b0[2] + bz), si);
+  }
+}
+// mark the (at most 8) blocks of one stencil
+__device__ void flag_stencil_blocks(const ddm_batch &B, const ddm_scene &S, int si, const int base[3]) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c)
+        mark_block(B, block_of(S, (base[0] + 2 * a) >> 2, (base[1] + 2 * b) >> 2, (base[2] + 2 * c) >> 2), si);
+}
+
+// ---------------------------------------------------------------- grid helpers
+struct BlockCoord {
+  int ix, iy, iz;
+};
+__device__ __forceinline__ BlockCoord block_node(const ddm_scene &S, int blk, int lane) {
+  int lb = blk - S.blk_begin;
+  int nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
+  int bx = lb / (nby * nbz), by = (lb / nbz) % nby, bz = lb % nbz;
+  return BlockCoord{bx * 4 + (lane >> 4), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane & 3)};
+}
+__device__ __forceinline__ NodeParams scene_node_params(const ddm_scene &S, const ddm_state &cur,
+                                                        const ddm_state &next) {
+  NodeParams P;
+  P.dim[0] = S.dim[0];
+  P.dim[1] = S.dim[1];
+  P.dim[2] = S.dim[2];
+  P.nb = S.n_bodies;
+  P.dx = S.dx;
+  P.dt = S.dt;
+  P.gf = S.ground_friction;
+  P.gravity = f3{S.gravity[0], S.gravity[1], S.gravity[2]};
+  P.prim = S.body_prim;
+  P.args = S.body_args;
+  P.pos = cur.body_pos;
+  P.rot = cur.body_rot;
+  P.npos = next.body_pos;
+  P.nrot = next.body_rot;
+  return P;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+struct NullScatter {
+  __device__ void operator()(int, int, int, f3) const {}
+};
+
+inline int grid_wgs(const ddm_batch *b) {
+  int w = (b->n_blocks + (kWG / 64) - 1) / (kWG / 64);
+  return w < 4096 ? (w > 0 ? w : 1) : 4096;
+}
+
+}  // namespace
+
+// Adjoint kernels (fused_adjoint.hip, compiled with FMA contraction: they feed only
+// gradients, whose bar is 1e-3, while the forward kernels keep the operation order of the
+// CPU restatement for 1e-5 state parity).
+void launch_g2p_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
+                    const ddm_state *next_grad, hipStream_t stream);
+void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
+                     const ddm_state *next_grad, hipStream_t stream);
+void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
+                    hipStream_t stream);
+
+}  // namespace ddm_fused
