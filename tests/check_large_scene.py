@@ -10,7 +10,7 @@ coordinates as one JSON line.
 Gradient check: the FP64 oracle cannot run 16 rollouts of 500 substeps over 10^6 particles
 in reasonable time, so the check is against FP32 central differences of the GPU rollout
 itself (eps = 0.1 in action units, i.e. joint-angle changes below 7e-4 rad), per
-coordinate and along the gradient direction. Their rounding noise (FP32 state, atomic
+coordinate and along the gradient direction for a sweep of step sizes. Their rounding noise (FP32 state, atomic
 order) is reported beside the error: the same kernels are checked against FP64 finite differences
 at the smaller hand-contact size by tests/test_gpu_hand.py.
 """
@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--checkpoint", type=int, default=25)
     ap.add_argument("--coords", type=int, default=8)
     ap.add_argument("--eps", type=float, default=0.1)
+    ap.add_argument("--no-tape", action="store_true", help="recompute adjoint without a segment tape")
+    ap.add_argument("--dir-eps", default="0.1,0.03,0.01,0.003,0.001",
+                    help="step sizes of the directional finite differences along the gradient")
     a = ap.parse_args()
     dev = torch.device("cuda:0")
     sc, tree, q0 = W.large_scene(a.particles)
@@ -48,7 +51,7 @@ def main():
     acts = W.hand_actions(a.env_steps)
     target = torch.tensor(W.plain_target(sc.x0), device=dev)
     eng = Engine([sc.spec], device=dev)
-    ro = Rollout(eng, T, checkpoint_every=a.checkpoint)
+    ro = Rollout(eng, T, checkpoint_every=a.checkpoint, tape=not a.no_tape)
     ro.set_initial(torch.tensor(sc.x0, device=dev))
     treed, q0d = tree.to(device=dev), q0.to(dev)
 
@@ -86,14 +89,19 @@ def main():
     # so the best signal-to-noise ratio a finite difference of this rollout can have
     gn = np.linalg.norm(g)
     dirv = (g / gn).astype(np.float32)
-    dd = (loss_of(acts + a.eps * dirv) - loss_of(acts - a.eps * dirv)) / (2 * a.eps)
+    sweep = {}
+    for h in [float(x) for x in a.dir_eps.split(",")]:
+        sweep[h] = (loss_of(acts + h * dirv) - loss_of(acts - h * dirv)) / (2 * h)
+        print(f"directional FD eps {h:g}: {sweep[h]:.6e} (adjoint {gn:.6e})", file=sys.stderr, flush=True)
+    dd = sweep[min(sweep)]
     out = {"particles": sc.n, "grid": 128, "substeps": T, "checkpoint_every": a.checkpoint, "bodies": sc.nb,
+           "adjoint": "recompute" if a.no_tape else "taped segments",
            "loss": loss, "fwd_bwd_seconds": el, "particle_substeps_per_s": sc.n * T / el,
            "peak_device_bytes": int(torch.cuda.max_memory_allocated()),
            "state_bytes": ro.state_bytes(), "coords": coords, "grad": gpick.tolist(), "fd_fp32": fd,
            "grad_vs_fd_fp32": rel_err(gpick, np.array(fd)),
-           "grad_norm": float(gn), "directional_fd_fp32": float(dd),
-           "directional_rel_err": float(abs(dd - gn) / gn),
+           "grad_norm": float(gn), "directional_fd_fp32": {str(h): float(v) for h, v in sweep.items()},
+           "directional_rel_err": {str(h): float(abs(v - gn) / gn) for h, v in sweep.items()},
            "loss_run_to_run_spread": float(np.ptp(l0)), "fd_noise_estimate": float(np.ptp(l0) / (2 * a.eps))}
     print(json.dumps(out), flush=True)
 
