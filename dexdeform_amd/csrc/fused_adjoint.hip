@@ -147,12 +147,23 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   const int G = build_groups(valid, key, min(kWG, ch.z - ch.y), sh.grp, gr, pos);
   if (valid) stage_record(rc, pos, st, 0.f, g_nv, gC);
   const float inv_dx = S.inv_dx;
-  auto contrib = [inv_dx](const StencilRecs &r, int s, int i, int j, int k, float w, float *acc) {
-    float4 a = r.f0[s];
-    f3 g = g2p_adj_contrib(w, f3{a.x, a.y, a.z}, rec_vec(r, s), rec_mat(r, s), inv_dx, i, j, k);
-    acc[0] += g.x;
-    acc[1] += g.y;
-    acc[2] += g.z;
+  // contributions of record s to the nodes (i, j, 0..2): g2p_adj_contrib's operations, with
+  // the k-independent ones done once (C adjoint = {b.w, c.x, c.y | c.z, c.w, d.x | d.y, d.z,
+  // d.w}, node-velocity adjoint seed = b.xyz)
+  auto contrib = [inv_dx](const StencilRecs &r, int s, int i, int j, float (*acc)[3]) {
+    const float4 a = r.f0[s], b = r.f1[s], c = r.f2[s], d = r.f3[s];
+    const float wxy = r.w[9 * s + i] * r.w[9 * s + 3 + j];
+    const float d0 = (float)i - a.x, d1 = (float)j - a.y;
+    const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float w = wxy * r.w[9 * s + 6 + k];
+      const float d2 = (float)k - a.z;
+      const float xx = w * inv_dx * 4.f;
+      acc[k][0] += w * b.x + (p0 + c.y * d2) * xx;
+      acc[k][1] += w * b.y + (p1 + d.x * d2) * xx;
+      acc[k][2] += w * b.z + (p2 + d.w * d2) * xx;
+    }
   };
   if (slabs_fit<kTileG2P>(T)) {
     const int sx = slab_width<kTileG2P>(T);
@@ -161,7 +172,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
       if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
       for (int i = threadIdx.x; i < L.n; i += kWG) tx[i] = ty[i] = tz[i] = 0.f;
       __syncthreads();  // also publishes the staged records
-      group_tasks<3>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
+      group_column_tasks<3>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
         const int li = L.local(X, Y, Z);
         atomicAdd(&tx[li], a[0]);
         atomicAdd(&ty[li], a[1]);
@@ -181,7 +192,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
     }
   } else {
     __syncthreads();  // publishes the staged records
-    group_tasks<3>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
+    group_column_tasks<3>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
       float *g = B.grid_dv + 4 * (size_t)node_index(S, X, Y, Z);
       atomicAdd(g + 0, a[0]);
       atomicAdd(g + 1, a[1]);

@@ -211,27 +211,29 @@ __device__ int build_groups(bool valid, int key, int n_valid, GroupScratch &sc, 
   return G;
 }
 
-// Every (group, stencil node) task whose node lies in slab L (all tasks when L is null):
-// acc = sum over the group's records of contrib(rc, slot, i, j, k, w), then
-// sink(X, Y, Z, acc). Tasks are spread over the workgroup's threads.
+// Every (group, i, j) task -- the column of 3 stencil nodes (i, j, 0..2) of a group -- whose
+// nodes lie in slab L (all tasks when L is null): acc[k] = sum over the group's records of
+// the contribution to node (i, j, k), then sink(X, Y, Z, acc[k]) for the 3 nodes. The
+// column shares each record read, the weight product w_x w_y and the x and y terms of the
+// affine product between its nodes; contrib(rc, slot, i, j, acc) adds one record's three
+// contributions with the same operations, in the same order, as per-node code would.
 template <int NC, class Contrib, class Sink>
-__device__ void group_tasks(int G, const Groups &gr, const StencilRecs &rc, const Tile *L, Contrib contrib,
-                            Sink sink) {
-  for (int task = (int)threadIdx.x; task < 27 * G; task += kWG) {
-    const int g = task / 27, o = task - 27 * (task / 27);
-    int i, j, k;
-    offset_ijk(o, i, j, k);
+__device__ void group_column_tasks(int G, const Groups &gr, const StencilRecs &rc, const Tile *L, Contrib contrib,
+                                   Sink sink) {
+  for (int task = (int)threadIdx.x; task < 9 * G; task += kWG) {
+    const int g = task / 9, ij = task - 9 * (task / 9);
+    const int i = ij / 3, j = ij - 3 * (ij / 3);
     int b[3];
     unpack_base(gr.key[g], b);
     if (L && !L->has_x(b[0] + i)) continue;
-    float acc[NC];
+    float acc[3][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = 0.f;
-    for (int q = gr.off[g], qe = gr.off[g + 1]; q < qe; ++q) {
-      const float w = rc.w[9 * q + i] * rc.w[9 * q + 3 + j] * rc.w[9 * q + 6 + k];  // weight_dyn's order
-      contrib(rc, q, i, j, k, w, acc);
-    }
-    sink(b[0] + i, b[1] + j, b[2] + k, acc);
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[k][c] = 0.f;
+    for (int q = gr.off[g], qe = gr.off[g + 1]; q < qe; ++q) contrib(rc, q, i, j, acc);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sink(b[0] + i, b[1] + j, b[2] + k, acc[k]);
   }
 }
 

@@ -55,15 +55,23 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
   const int G = build_groups(valid, key, min(kWG, ch.z - ch.y), sh.grp, gr, pos);
   if (valid) stage_record(rc, pos, q.st, q.m, q.mv, q.affine);
   const float dx = S.dx;
-  auto contrib = [dx](const StencilRecs &r, int s, int i, int j, int k, float w, float *acc) {
-    float cm;
-    f3 cmv;
-    float4 a = r.f0[s];
-    p2g_node_contrib(w, f3{a.x, a.y, a.z}, a.w, rec_vec(r, s), rec_mat(r, s), dx, i, j, k, cm, cmv);
-    acc[0] += cm;
-    acc[1] += cmv.x;
-    acc[2] += cmv.y;
-    acc[3] += cmv.z;
+  // contributions of record s to the nodes (i, j, 0..2): p2g_node_contrib's operations,
+  // with the k-independent ones done once (affine = {b.w, c.x, c.y | c.z, c.w, d.x | d.y,
+  // d.z, d.w}, rows summed left to right as mvec does)
+  auto contrib = [dx](const StencilRecs &r, int s, int i, int j, float (*acc)[4]) {
+    const float4 a = r.f0[s], b = r.f1[s], c = r.f2[s], d = r.f3[s];
+    const float wxy = r.w[9 * s + i] * r.w[9 * s + 3 + j];
+    const float d0 = ((float)i - a.x) * dx, d1 = ((float)j - a.y) * dx;
+    const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float w = wxy * r.w[9 * s + 6 + k];
+      const float d2 = ((float)k - a.z) * dx;
+      acc[k][0] += a.w * w;
+      acc[k][1] += (b.x + (p0 + c.y * d2)) * w;
+      acc[k][2] += (b.y + (p1 + d.x * d2)) * w;
+      acc[k][3] += (b.z + (p2 + d.w * d2)) * w;
+    }
   };
   float *gmmv = B.grid_mmv;
   if (slabs_fit<kTileP2G>(T)) {
@@ -73,7 +81,7 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
       if (xs != T.lo[0]) __syncthreads();  // the previous pass has flushed the tile
       for (int i = threadIdx.x; i < L.n; i += kWG) tm[i] = tx[i] = ty[i] = tz[i] = 0.f;
       __syncthreads();  // also publishes the staged records
-      group_tasks<4>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
+      group_column_tasks<4>(G, gr, rc, &L, contrib, [&](int X, int Y, int Z, const float *a) {
         const int li = L.local(X, Y, Z);
         atomicAdd(&tm[li], a[0]);
         atomicAdd(&tx[li], a[1]);
@@ -96,7 +104,7 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
     flag_tile_blocks(B, S, ch.x, T);
   } else {
     __syncthreads();  // publishes the staged records
-    group_tasks<4>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
+    group_column_tasks<4>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
       float *g = gmmv + 4 * (size_t)node_index(S, X, Y, Z);
       atomicAdd(g + 0, a[0]);
       atomicAdd(g + 1, a[1]);
