@@ -167,8 +167,7 @@ class KernelTimer:
             evs, arr = events(4)
             if taped:
                 _lib.check(lib.ddm_substep_forward_taped_timed(
-                    ctypes.byref(batches[t]), ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(),
-                    ctypes.byref(ro.tape.tapes[t]), st, arr))
+                    ctypes.byref(batches[t]), ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(), st, arr))
             else:
                 _lib.check(lib.ddm_substep_forward_timed(ctypes.byref(eng._batch), ro._sview(t, t).data_ptr(),
                                                          ro._sview(t + 1, t + 1).data_ptr(), st, arr))
@@ -188,8 +187,7 @@ class KernelTimer:
                     ro._gview(nxt, t + 1).data_ptr())
             if taped:
                 evs, arr = events(5)
-                _lib.check(lib.ddm_substep_backward_taped_timed(ctypes.byref(batches[t]), *tabs,
-                                                                ctypes.byref(ro.tape.tapes[t]), st, arr))
+                _lib.check(lib.ddm_substep_backward_taped_timed(ctypes.byref(batches[t]), *tabs, st, arr))
             else:
                 evs, arr = events(6)
                 _lib.check(lib.ddm_substep_backward_timed(ctypes.byref(eng._batch), *tabs, st, arr))

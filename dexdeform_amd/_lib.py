@@ -35,6 +35,8 @@ class DdmScene(ctypes.Structure):
         ("ground_height", c_float), ("gravity", c_float * 3),
         ("mass", c_void_p), ("vol", c_void_p), ("mu_lam_yield", c_void_p), ("body_prim", c_void_p),
         ("body_args", c_void_p), ("tmp_Ft", c_void_p), ("tmp_U", c_void_p), ("tmp_V", c_void_p), ("tmp_sig", c_void_p),
+        ("tape_mmv", c_void_p), ("tape_v", c_void_p), ("tape_list", c_void_p), ("tape_status", c_void_p),
+        ("tape_capacity", c_int32), ("reserved", c_int32),
     ]
 
 
@@ -45,8 +47,10 @@ SCENE_DTYPE = np.dtype([
     ("ground_height", "<f4"), ("gravity", "<f4", (3,)),
     ("mass", "<u8"), ("vol", "<u8"), ("mu_lam_yield", "<u8"), ("body_prim", "<u8"), ("body_args", "<u8"),
     ("tmp_Ft", "<u8"), ("tmp_U", "<u8"), ("tmp_V", "<u8"), ("tmp_sig", "<u8"),
+    ("tape_mmv", "<u8"), ("tape_v", "<u8"), ("tape_list", "<u8"), ("tape_status", "<u8"),
+    ("tape_capacity", "<i4"), ("reserved", "<i4"),
 ])
-assert SCENE_DTYPE.itemsize == ctypes.sizeof(DdmScene) == 136
+assert SCENE_DTYPE.itemsize == ctypes.sizeof(DdmScene) == 176
 
 
 class DdmBatch(ctypes.Structure):
@@ -56,14 +60,6 @@ class DdmBatch(ctypes.Structure):
         ("grid_dv", c_void_p), ("grid_dmmv", c_void_p), ("block_flags", c_void_p), ("block_list", c_void_p),
         ("block_count", c_void_p),
     ]
-
-
-class DdmGridTape(ctypes.Structure):
-    _fields_ = [("grid_mmv", c_void_p), ("grid_v", c_void_p), ("block_list", c_void_p), ("status", c_void_p),
-                ("capacity", c_int32), ("reserved", c_int32)]
-
-
-assert ctypes.sizeof(DdmGridTape) == 40
 
 
 class DdmState(ctypes.Structure):
@@ -104,11 +100,10 @@ _ARGTYPES = {
     "ddm_substep_backward": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P],
     "ddm_substep_forward_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P],
     "ddm_substep_backward_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
-    "ddm_substep_forward_taped": [ctypes.POINTER(DdmBatch), _P, _P, ctypes.POINTER(DdmGridTape), _P],
-    "ddm_substep_backward_taped": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, ctypes.POINTER(DdmGridTape), _P],
-    "ddm_substep_forward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, ctypes.POINTER(DdmGridTape), _P, _P],
-    "ddm_substep_backward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, ctypes.POINTER(DdmGridTape), _P,
-                                         _P],
+    "ddm_substep_forward_taped": [ctypes.POINTER(DdmBatch), _P, _P, _P],
+    "ddm_substep_backward_taped": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P],
+    "ddm_substep_forward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P],
+    "ddm_substep_backward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
     "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],
     "ddm_version": [],
 }

@@ -7,16 +7,16 @@ namespace ddm_fused {
 namespace {
 
 // ---------------------------------------------------------------- P2G (+ F advection, SVD, return map)
-enum P2GMode { kForward = 0, kRecompute = 1, kForwardTaped = 2 };
-// kForward     : forward substep, writes next.F.
-// kRecompute   : adjoint recompute, writes the F_tmp/U/V/sigma temporaries instead.
-// kForwardTaped: writes both (the taped adjoint reads the temporaries).
-// need (optional): the kernel runs only if *need != 0 (a taped adjoint that has to
-// recompute decides this on the device).
+enum P2GMode { kForward = 0, kRecompute = 1, kForwardTaped = 2, kRecomputeUntaped = 3 };
+// kForward         : forward substep, writes next.F.
+// kRecompute       : adjoint recompute, writes the F_tmp/U/V/sigma temporaries instead.
+// kForwardTaped    : writes next.F, and the temporaries of scenes that keep a snapshot
+//                    (the taped adjoint reads them).
+// kRecomputeUntaped: kRecompute for the scenes whose snapshot was not restored (no
+//                    snapshot, or tape_status[1] set by k_restore); other chunks exit.
 template <int MODE>
 __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                             const ddm_state *__restrict__ next_tab, const int32_t *need) {
-  if (need && *need == 0) return;  // uniform over the grid
+                                             const ddm_state *__restrict__ next_tab) {
   __shared__ ScatterLDS<4, kTileP2G> sh;
   __shared__ StencilRecs rc;
   __shared__ Groups gr;
@@ -24,6 +24,8 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
   float *const tm = sh.tile[0], *const tx = sh.tile[1], *const ty = sh.tile[2], *const tz = sh.tile[3];
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
+  if (MODE == kRecomputeUntaped && S.tape_mmv && S.tape_status[1] == 0) return;  // uniform: one scene per chunk
+  const bool temps = MODE == kRecompute || MODE == kRecomputeUntaped || (MODE == kForwardTaped && S.tape_mmv);
   const ddm_state cur = cur_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   const bool valid = p < ch.z;
@@ -39,13 +41,13 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
     m33 Fn;
     p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                  ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
-    if (MODE != kForward) {
+    if (temps) {
       st9(S.tmp_Ft + 9 * (size_t)p, Ft);
       st9(S.tmp_U + 9 * (size_t)p, U);
       st9(S.tmp_V + 9 * (size_t)p, V);
       st3(S.tmp_sig + 3 * (size_t)p, sig);
     }
-    if (MODE != kRecompute) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
+    if (MODE == kForward || MODE == kForwardTaped) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     key = pack_base(q.st.base);
   } else {
     q.st.base[0] = q.st.base[1] = q.st.base[2] = 0;
@@ -121,24 +123,24 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
 
 // ---------------------------------------------------------------- grid update
 // ZERO=true (forward substep): consumes (m, mv) and clears them and the flags.
-// SAVE=true (taped forward): also copies each listed block's (m, mv) and v, and the list,
-// into the tape (entries beyond its capacity are dropped; status[0] keeps the count).
-// need (optional): run only if *need != 0.
-template <bool ZERO, bool SAVE>
+// SAVE=true (taped forward): also copies each listed block's (m, mv) and v into its
+//   scene's snapshot, if it keeps one (blocks beyond its capacity are dropped; the count in
+//   tape_status[0] keeps growing, so the taped adjoint sees the overflow).
+// SKIP_RESTORED=true (taped adjoint's recompute): skips the blocks of scenes whose
+//   snapshot was restored.
+template <bool ZERO, bool SAVE, bool SKIP_RESTORED>
 __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                              const ddm_state *__restrict__ next_tab, ddm_grid_tape tape,
-                                              const int32_t *need) {
-  if (need && *need == 0) return;  // uniform over the grid
+                                              const ddm_state *__restrict__ next_tab) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
   const int n_listed = *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
-  if (SAVE && blockIdx.x == 0 && threadIdx.x == 0) tape.status[0] = n_listed;
   for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
     const int2 entry = list[e];
     const int blk = entry.x, si = entry.y;
     const ddm_scene &S = B.scenes[si];
+    if (SKIP_RESTORED && S.tape_mmv && S.tape_status[1] == 0) continue;
     BlockCoord c = block_node(S, blk, lane);
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
     float4 a = mmv[gi];
@@ -148,11 +150,16 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
       v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P);
     }
     gv[gi] = make_float4(v.x, v.y, v.z, 0.f);
-    if (SAVE && e < tape.capacity) {
-      const size_t ti = ((size_t)e << 6) | (size_t)lane;
-      reinterpret_cast<float4 *>(tape.grid_mmv)[ti] = a;
-      reinterpret_cast<float4 *>(tape.grid_v)[ti] = make_float4(v.x, v.y, v.z, 0.f);
-      if (lane == 0) reinterpret_cast<int2 *>(tape.block_list)[e] = entry;
+    if (SAVE && S.tape_mmv) {
+      int slot = 0;
+      if (lane == 0) slot = atomicAdd(S.tape_status, 1);
+      slot = __shfl(slot, 0, 64);
+      if (slot < S.tape_capacity) {
+        const size_t ti = ((size_t)slot << 6) | (size_t)lane;
+        reinterpret_cast<float4 *>(S.tape_mmv)[ti] = a;
+        reinterpret_cast<float4 *>(S.tape_v)[ti] = make_float4(v.x, v.y, v.z, 0.f);
+        if (lane == 0) S.tape_list[slot] = blk;
+      }
     }
     if (ZERO) {
       mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -161,25 +168,58 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
   }
 }
 
-// Taped adjoint: puts a substep's grid back into the arena from its tape, or, when the
-// forward listed more blocks than the tape holds, leaves the (clear) arena alone and sets
-// status[1] so that the recompute kernels run.
-__global__ __launch_bounds__(kWG) void k_restore(ddm_batch B, ddm_grid_tape tape) {
-  const int n = tape.status[0];
-  const bool use = n <= tape.capacity;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    tape.status[1] = use ? 0 : 1;
-    if (use) *B.block_count = n;
-  }
-  if (!use) return;
+// Taped forward, first: empties every snapshot of the launch (scenes without particles
+// included, so that no stale snapshot survives).
+__global__ __launch_bounds__(kWG) void k_tape_reset(ddm_batch B) {
+  for (int si = threadIdx.x; si < B.n_scenes; si += kWG)
+    if (B.scenes[si].tape_mmv) B.scenes[si].tape_status[0] = 0;
+}
+
+// blocks a scene restores from its snapshot (0 without one, or when it overflowed)
+__device__ __forceinline__ int restorable(const ddm_scene &S) {
+  if (!S.tape_mmv) return 0;
+  const int n = S.tape_status[0];
+  return n <= S.tape_capacity ? n : 0;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Taped adjoint: for every scene with a snapshot that holds all its listed blocks, puts
+// the blocks back into the arena and into the block list (scenes in order, each at the
+// prefix sum of the restored counts before it), and clears tape_status[1]; a scene whose
+// forward listed more blocks than its snapshot holds gets tape_status[1] = 1 and is
+// recomputed by the kernels that follow (its arena blocks are untouched, i.e. clear; its
+// recomputed blocks are appended after the restored ones). Grid: (kRestoreWGs, n_scenes),
+// one row of workgroups per scene.
+constexpr int kRestoreWGs = 16;
+__global__ __launch_bounds__(kWG) void k_restore(ddm_batch B) {
+  const int si = (int)blockIdx.y;
+  const ddm_scene &S = B.scenes[si];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int2 *saved = reinterpret_cast<const int2 *>(tape.block_list);
+  // restored entries of the scenes before this one, and of all scenes
+  int before = 0, total = 0;
+  for (int s0 = 0; s0 < B.n_scenes; s0 += 64) {
+    const int s2 = s0 + lane;
+    const int c = s2 < B.n_scenes ? restorable(B.scenes[s2]) : 0;
+    total += wave_sum_i(c);
+    before += wave_sum_i(s2 < si ? c : 0);
+  }
+  if (si == 0 && blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = total;
+  if (!S.tape_mmv) return;
+  const int n = S.tape_status[0];
+  const bool use = n <= S.tape_capacity;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.tape_status[1] = use ? 0 : 1;
+  if (!use) return;
+  int2 *list = reinterpret_cast<int2 *>(B.block_list) + before;
   for (int e = blockIdx.x * (kWG / 64) + wave; e < n; e += gridDim.x * (kWG / 64)) {
-    const int2 entry = saved[e];
-    const size_t gi = ((size_t)entry.x << 6) | (size_t)lane, ti = ((size_t)e << 6) | (size_t)lane;
-    reinterpret_cast<float4 *>(B.grid_mmv)[gi] = reinterpret_cast<const float4 *>(tape.grid_mmv)[ti];
-    reinterpret_cast<float4 *>(B.grid_v)[gi] = reinterpret_cast<const float4 *>(tape.grid_v)[ti];
-    if (lane == 0) reinterpret_cast<int2 *>(B.block_list)[e] = entry;
+    const int blk = S.tape_list[e];
+    const size_t gi = ((size_t)blk << 6) | (size_t)lane, ti = ((size_t)e << 6) | (size_t)lane;
+    reinterpret_cast<float4 *>(B.grid_mmv)[gi] = reinterpret_cast<const float4 *>(S.tape_mmv)[ti];
+    reinterpret_cast<float4 *>(B.grid_v)[gi] = reinterpret_cast<const float4 *>(S.tape_v)[ti];
+    if (lane == 0) list[e] = make_int2(blk, si);
   }
 }
 
@@ -236,9 +276,9 @@ static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_st
   if (e) return e;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, nullptr);
+  k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, ddm_grid_tape{}, nullptr);
+  k_grid<true, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
@@ -252,9 +292,9 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
   if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, nullptr);
+  k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, ddm_grid_tape{}, nullptr);
+  k_grid<false, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 2, stream);
   launch_g2p_adj(*b, cur, next, cur_grad, next_grad, stream);
   mark(ev, 3, stream);
@@ -265,21 +305,16 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
   return (int)hipGetLastError();
 }
 
-static inline int check_tape(const ddm_grid_tape *t) {
-  if (!t || !t->grid_mmv || !t->grid_v || !t->block_list || !t->status || t->capacity < 0) return DDM_EINVAL;
-  return 0;
-}
-
 static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *ev) {
+                                hipStream_t stream, hipEvent_t *ev) {
   int e = check(b, cur, next);
-  if (!e) e = check_tape(tape);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, nullptr);
+  k_tape_reset<<<1, kWG, 0, stream>>>(*b);
+  k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<true, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, *tape, nullptr);
+  k_grid<true, true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
@@ -287,18 +322,16 @@ static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const 
 }
 
 static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                 const ddm_state *cur_grad, const ddm_state *next_grad, const ddm_grid_tape *tape,
-                                 hipStream_t stream, hipEvent_t *ev) {
+                                 const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
+                                 hipEvent_t *ev) {
   int e = check(b, cur, next);
-  if (!e) e = check_tape(tape);
   if (e) return e;
   if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
-  const int32_t *need = tape->status + 1;
   mark(ev, 0, stream);
-  k_restore<<<grid_wgs(b), kWG, 0, stream>>>(*b, *tape);
-  k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, need);
-  k_grid<false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, ddm_grid_tape{}, need);
+  k_restore<<<dim3(kRestoreWGs, b->n_scenes), kWG, 0, stream>>>(*b);
+  k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
   launch_g2p_adj(*b, cur, next, cur_grad, next_grad, stream);
   mark(ev, 2, stream);
@@ -310,27 +343,26 @@ static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const
 }
 
 DDM_API int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                      const ddm_grid_tape *tape, hipStream_t stream) {
-  return launch_forward_taped(b, cur, next, tape, stream, nullptr);
+                                      hipStream_t stream) {
+  return launch_forward_taped(b, cur, next, stream, nullptr);
 }
 
 DDM_API int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                       const ddm_state *cur_grad, const ddm_state *next_grad,
-                                       const ddm_grid_tape *tape, hipStream_t stream) {
-  return launch_backward_taped(b, cur, next, cur_grad, next_grad, tape, stream, nullptr);
+                                       const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
+  return launch_backward_taped(b, cur, next, cur_grad, next_grad, stream, nullptr);
 }
 
 DDM_API int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                            const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events) {
+                                            hipStream_t stream, hipEvent_t *events) {
   if (!events) return DDM_EINVAL;
-  return launch_forward_taped(b, cur, next, tape, stream, events);
+  return launch_forward_taped(b, cur, next, stream, events);
 }
 
 DDM_API int ddm_substep_backward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                              const ddm_state *cur_grad, const ddm_state *next_grad,
-                                             const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events) {
+                                             hipStream_t stream, hipEvent_t *events) {
   if (!events) return DDM_EINVAL;
-  return launch_backward_taped(b, cur, next, cur_grad, next_grad, tape, stream, events);
+  return launch_backward_taped(b, cur, next, cur_grad, next_grad, stream, events);
 }
 
 DDM_API int ddm_substep_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream) {

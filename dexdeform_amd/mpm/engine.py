@@ -292,21 +292,38 @@ class Engine:
                    "ddm_substep_backward")
 
     # ------------------------------------------------------------------ taped adjoint
-    def temps_table(self, temps: torch.Tensor) -> torch.Tensor:
-        """Scene table whose F_tmp/U/V/sigma temporaries live in ``temps`` (a float32
-        tensor of 30 * n_particles_total elements: F_tmp | U | V | sigma), so that a taped
-        substep keeps its own copy for its adjoint."""
+    def tape_capacities(self) -> List[int]:
+        """Default snapshot size per scene (blocks): every block of a small grid, else
+        particles / 32 + 64 (a substep that lists more is recomputed by its adjoint)."""
+        blocks = [(int(s.grid_dim[0]) // 4) * (int(s.grid_dim[1]) // 4) * (int(s.grid_dim[2]) // 4) for s in self.scenes]
+        return [min(b, int(c) // 32 + 64) for b, c in zip(blocks, self.capacity)]
+
+    def taped_table(self, temps, tape_mmv, tape_v, tape_list, tape_status, caps) -> torch.Tensor:
+        """Scene table of one taped substep: each scene's F_tmp/U/V/sigma temporaries in
+        ``temps`` (30 * n_particles_total floats: F_tmp | U | V | sigma) and its grid
+        snapshot in the next caps[i] blocks of tape_mmv / tape_v / tape_list (scenes in
+        order), with two status ints per scene in ``tape_status``."""
         P = self.n_particles_total
         if temps.numel() < 30 * P or temps.dtype != torch.float32 or not temps.is_contiguous():
             raise ValueError("temps must be a contiguous float32 tensor of 30 * n_particles_total elements")
+        need = int(sum(caps))
+        if tape_mmv.numel() < need * 256 or tape_v.numel() < need * 256 or tape_list.numel() < need:
+            raise ValueError("tape buffers smaller than the scenes' capacities")
         base, f4 = int(temps.data_ptr()), 4
         tab = self._scene_np.copy()
+        off = 0
         for i in range(len(tab)):
             pb = int(self.p_begin[i])
             tab[i]["tmp_Ft"] = base + 9 * pb * f4
             tab[i]["tmp_U"] = base + (9 * P + 9 * pb) * f4
             tab[i]["tmp_V"] = base + (18 * P + 9 * pb) * f4
             tab[i]["tmp_sig"] = base + (27 * P + 3 * pb) * f4
+            tab[i]["tape_mmv"] = int(tape_mmv.data_ptr()) + off * 256 * f4
+            tab[i]["tape_v"] = int(tape_v.data_ptr()) + off * 256 * f4
+            tab[i]["tape_list"] = int(tape_list.data_ptr()) + off * 4
+            tab[i]["tape_status"] = int(tape_status.data_ptr()) + 2 * i * 4
+            tab[i]["tape_capacity"] = int(caps[i])
+            off += int(caps[i])
         return torch.as_tensor(tab.view(np.uint8).copy(), device=self.device)
 
     def _batch_with_scenes(self, scene_tab):
@@ -314,21 +331,20 @@ class Engine:
         b.scenes = scene_tab.data_ptr()
         return b
 
-    def forward_taped(self, cur_tab, nxt_tab, scene_tab, tape, stream=None):
-        """Forward substep recording ``tape`` (a DdmGridTape) and, through ``scene_tab``
-        (from temps_table), the temporaries."""
+    def forward_taped(self, cur_tab, nxt_tab, scene_tab, stream=None):
+        """Forward substep recording, per scene, the snapshot and temporaries that the
+        scene table from taped_table points to."""
         b = self._batch_with_scenes(scene_tab)
         _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
-                                                      ctypes.byref(tape), _lib.stream_ptr(stream)),
-                   "ddm_substep_forward_taped")
+                                                      _lib.stream_ptr(stream)), "ddm_substep_forward_taped")
 
-    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, tape, stream=None):
-        """Adjoint substep from the tape recorded by forward_taped on the same tables."""
+    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, stream=None):
+        """Adjoint substep from what forward_taped recorded with the same scene table."""
         if not self.with_grad:
             raise RuntimeError("engine was created with with_grad=False")
         b = self._batch_with_scenes(scene_tab)
         _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
-                                                       gcur_tab.data_ptr(), gnxt_tab.data_ptr(), ctypes.byref(tape),
+                                                       gcur_tab.data_ptr(), gnxt_tab.data_ptr(),
                                                        _lib.stream_ptr(stream)), "ddm_substep_backward_taped")
 
     def reset_grid(self, stream=None):

@@ -55,22 +55,22 @@ class _Particles:
 
 class _Tapes:
     """Per-substep tapes of one rollout segment: the SVD temporaries (30 floats per
-    particle, through a scene table of their own) and a snapshot of the listed grid blocks
-    (up to ``capacity`` blocks; beyond it the adjoint recomputes that substep)."""
+    particle) and, per scene, a snapshot of its listed grid blocks (up to its capacity;
+    beyond it the adjoint recomputes that scene's substep), reached through a scene table
+    per slot."""
 
-    def __init__(self, engine: Engine, slots: int, capacity: int):
+    def __init__(self, engine: Engine, slots: int, caps):
         dev = engine.device
         P = engine.n_particles_total
-        self.capacity = int(capacity)
+        self.caps = [int(c) for c in caps]
+        nblk = sum(self.caps)
         self.temps = torch.empty(slots, 30 * P, dtype=torch.float32, device=dev)
-        self.scene_tabs = [engine.temps_table(self.temps[i]) for i in range(slots)]
-        self.grid_mmv = torch.empty(slots, self.capacity * 64 * 4, dtype=torch.float32, device=dev)
+        self.grid_mmv = torch.empty(slots, nblk * 256, dtype=torch.float32, device=dev)
         self.grid_v = torch.empty_like(self.grid_mmv)
-        self.block_list = torch.empty(slots, self.capacity * 2, dtype=torch.int32, device=dev)
-        self.status = torch.zeros(slots, 2, dtype=torch.int32, device=dev)
-        self.tapes = [_lib.DdmGridTape(self.grid_mmv[i].data_ptr(), self.grid_v[i].data_ptr(),
-                                       self.block_list[i].data_ptr(), self.status[i].data_ptr(), self.capacity, 0)
-                      for i in range(slots)]
+        self.block_list = torch.empty(slots, nblk, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(slots, len(self.caps), 2, dtype=torch.int32, device=dev)
+        self.scene_tabs = [engine.taped_table(self.temps[i], self.grid_mmv[i], self.grid_v[i], self.block_list[i],
+                                              self.status[i], self.caps) for i in range(slots)]
 
     def nbytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in (self.temps, self.grid_mmv, self.grid_v, self.block_list,
@@ -83,9 +83,9 @@ class Rollout:
         """tape=True keeps, for every substep of a segment, the SVD temporaries and the
         grid of the forward pass, so the adjoint substeps skip recomputing P2G and the
         grid update (120 B per particle plus 2 KiB per active grid block per substep).
-        tape_blocks bounds the blocks one substep's snapshot holds (default: all blocks
-        of the engine, capped at n_particles / 32 + 1024); a substep that lists more
-        is recomputed by its adjoint instead (decided on the device)."""
+        tape_blocks bounds the blocks one scene's snapshot holds per substep (an int for
+        every scene or a list per scene; default: Engine.tape_capacities); a scene whose substep lists more is recomputed by its
+        adjoint instead (decided on the device)."""
         self.engine = engine
         self.T = int(T)
         dev = engine.device
@@ -120,8 +120,13 @@ class Rollout:
         self._sview, self._gview = sview, gview
         self.tape = None
         if tape:
-            cap = tape_blocks or min(engine.n_blocks, engine.n_particles_total // 32 + 1024)
-            self.tape = _Tapes(engine, self.K, cap)
+            if tape_blocks is None:
+                caps = engine.tape_capacities()
+            elif isinstance(tape_blocks, (list, tuple)):
+                caps = [int(c) for c in tape_blocks]
+            else:
+                caps = [int(tape_blocks)] * len(engine.scenes)
+            self.tape = _Tapes(engine, self.K, caps)
 
     # ------------------------------------------------------------------ inputs / outputs
     def set_initial(self, x, v=None, F=None, C=None):
@@ -148,8 +153,7 @@ class Rollout:
     # ------------------------------------------------------------------ passes
     def _fwd(self, j, t, taped=False):
         if taped:
-            self.engine.forward_taped(self._sview(j, t), self._sview(j + 1, t + 1), self.tape.scene_tabs[j],
-                                      self.tape.tapes[j])
+            self.engine.forward_taped(self._sview(j, t), self._sview(j + 1, t + 1), self.tape.scene_tabs[j])
         else:
             self.engine.forward_tables(self._sview(j, t), self._sview(j + 1, t + 1))
 
@@ -196,7 +200,7 @@ class Rollout:
                 tabs = (self._sview(j, t), self._sview(j + 1, t + 1), self._gview(cur_slot, t),
                         self._gview(nxt_slot, t + 1))
                 if self.tape is not None:
-                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], self.tape.tapes[j])
+                    eng.backward_taped(*tabs, self.tape.scene_tabs[j])
                 else:
                     eng.backward_tables(*tabs)
         self._g0 = self.g[cur_slot]
@@ -237,5 +241,6 @@ class Rollout:
         return per * (len(self.ckpt) + len(self.seg) + len(self.g)) + (self.tape.nbytes() if self.tape else 0)
 
     def tape_recomputed(self) -> int:
-        """Substeps of the last backward segment that had to recompute (snapshot full)."""
-        return int(self.tape.status[:, 1].sum().item()) if self.tape is not None else 0
+        """(Scene, substep) pairs of the last backward segment that had to recompute
+        because the scene's snapshot was full."""
+        return int(self.tape.status[:, :, 1].sum().item()) if self.tape is not None else 0

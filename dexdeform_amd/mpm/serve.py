@@ -194,13 +194,20 @@ def _state_records(records) -> np.ndarray:
     return np.array(records, dtype=np.int64).reshape(-1, 6)
 
 
+def tape_capacity(req: Request) -> int:
+    """Grid blocks one substep's snapshot of the request holds (a substep that lists more
+    is recomputed by its adjoint)."""
+    gd = req.spec.grid_dim
+    return min((gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4), req.n // 32 + 64)
+
+
 def request_bytes(req: Request) -> int:
     """Device bytes a request holds while active."""
     T = req.T
     K = req.checkpoint_every or T
     n_states = (T // K + 1 if K != T else 0) + (K + 1) + 2
     per_state = req.n * 24 * 4
-    temps = req.n * (9 * 3 + 3 + 5) * 4
+    temps = req.n * 5 * 4 + K * (req.n * 30 * 4 + tape_capacity(req) * (2 * 1024 + 4) + 8)
     nb = max(req.spec.n_bodies, 1)
     poses = (T + 1) * nb * 7 * 4 * 2
     gd = req.spec.grid_dim
@@ -227,9 +234,14 @@ class _Active:
         self.mass, self.vol, self.mly = f(sp.mass), f(sp.vol), f(np.asarray(sp.mu_lam_yield).reshape(-1, 3))
         self.prim = f(np.asarray(sp.body_prim).reshape(-1, 4)) if nb else None
         self.args = f(np.asarray(sp.body_args).reshape(-1, 4)) if nb else None
-        tmp = torch.empty(30 * n, dtype=torch.float32, device=dev)  # written by the adjoint recompute
-        self.tmp = [tmp[0:9 * n].view(n, 9), tmp[9 * n:18 * n].view(n, 9), tmp[18 * n:27 * n].view(n, 9),
-                    tmp[27 * n:30 * n].view(n, 3)]
+        # taped adjoint: per substep of a segment, the SVD temporaries and a grid snapshot
+        # (written by the segment's taped forward pass, read by its adjoint substeps)
+        self.tape_cap = tape_capacity(req)
+        self.tape_temps = torch.empty(self.K, 30 * n, dtype=torch.float32, device=dev)
+        self.tape_mmv = torch.empty(self.K, self.tape_cap * 256, dtype=torch.float32, device=dev)
+        self.tape_v = torch.empty_like(self.tape_mmv)
+        self.tape_list = torch.empty(self.K, self.tape_cap, dtype=torch.int32, device=dev)
+        self.tape_status = torch.zeros(self.K, 2, dtype=torch.int32, device=dev)
         nb_rows = max(nb, 1)
         self.pose_pos = torch.zeros(T + 1, nb_rows, 3, device=dev)
         self.pose_rot = torch.zeros(T + 1, nb_rows, 4, device=dev)
@@ -268,7 +280,7 @@ class _Active:
         r["mass"], r["vol"], r["mu_lam_yield"] = self.mass.data_ptr(), self.vol.data_ptr(), self.mly.data_ptr()
         r["body_prim"] = self.prim.data_ptr() if nb else 0
         r["body_args"] = self.args.data_ptr() if nb else 0
-        r["tmp_Ft"], r["tmp_U"], r["tmp_V"], r["tmp_sig"] = (t.data_ptr() for t in self.tmp)
+        self._temps_of(r, 0)  # untaped forward substeps write no temporaries; any slot will do
         self.chunks = chunk_table([n]).astype(np.int32)  # (0, first, end, 0) rows; column 0 = scene slot
         self.ops = self._schedule()
         self._addresses()
@@ -276,9 +288,14 @@ class _Active:
         self.done_event = None
         self.loss = None
 
+    def _temps_of(self, r, j):
+        base, n, f4 = int(self.tape_temps[j].data_ptr()), self.req.n, 4
+        r["tmp_Ft"], r["tmp_U"], r["tmp_V"], r["tmp_sig"] = base, base + 9 * n * f4, base + 18 * n * f4, base + 27 * n * f4
+
     # ---- operation schedule: ("pre", callable) run before the tick's launches,
-    #      ("F", j, t) forward substep seg[j] -> seg[j+1] at substep t,
-    #      ("B", j, t, gcur, gnxt) adjoint substep.
+    #      ("F", j, t, taped) forward substep seg[j] -> seg[j+1] at substep t (taped: it
+    #      records slot j's tape for the adjoint), ("B", j, t, gcur, gnxt) adjoint substep
+    #      from slot j's tape.
     def _schedule(self):
         T, K = self.req.T, self.K
         ops: List[tuple] = []
@@ -287,7 +304,7 @@ class _Active:
             if not self.full and c > 0:
                 ops.append(("pre", lambda c=c: self.seg[0].copy_(self.ckpt[c]), c))
             for j in range(K):
-                ops.append(("F", j, c * K + j))
+                ops.append(("F", j, c * K + j, self.full))
             if not self.full:
                 ops.append(("post", lambda c=c: self.ckpt[c + 1].copy_(self.seg[K]), c))
         ops.append(("loss", None, None))
@@ -296,7 +313,7 @@ class _Active:
             if not self.full:
                 ops.append(("pre", lambda c=c: self.seg[0].copy_(self.ckpt[c]), c))
                 for j in range(K):
-                    ops.append(("F", j, c * K + j))
+                    ops.append(("F", j, c * K + j, True))
             for j in range(K - 1, -1, -1):
                 nxt, cur = cur, 1 - cur
                 ops.append(("B", j, c * K + j, cur, nxt))
@@ -329,6 +346,19 @@ class _Active:
         self.pose_addr = (int(self.pose_pos.data_ptr()), int(self.pose_rot.data_ptr()), rows * 12, rows * 16)
         self.pgrad_addr = (int(self.pos_grad.data_ptr()), int(self.rot_grad.data_ptr()))
         self.rec_bytes = np.array([self.record], dtype=_lib.SCENE_DTYPE).view(np.uint8)
+        self.rec_taped = []
+        for j in range(self.K):
+            r = self.record.copy()
+            self._temps_of(r, j)
+            r["tape_mmv"], r["tape_v"] = int(self.tape_mmv[j].data_ptr()), int(self.tape_v[j].data_ptr())
+            r["tape_list"], r["tape_status"] = int(self.tape_list[j].data_ptr()), int(self.tape_status[j].data_ptr())
+            r["tape_capacity"] = self.tape_cap
+            self.rec_taped.append(np.array([r], dtype=_lib.SCENE_DTYPE).view(np.uint8))
+
+    def record_for(self, op):
+        """Scene record of a kernel operation: slot j's taped record, or the plain one."""
+        taped = op[0] == "B" or op[3]
+        return self.rec_taped[op[1]] if taped else self.rec_bytes
 
     def state_record(self, j, t):
         pp, pr, sp, sr = self.pose_addr
@@ -414,7 +444,7 @@ class RolloutServer:
     # ------------------------------------------------------------------ launches
     def _launch(self, entries, kind):
         """entries: list of (active, op). One fused launch sequence for all of them."""
-        recs = np.concatenate([a.rec_bytes for a, _ in entries])
+        recs = np.concatenate([a.record_for(op) for a, op in entries])
         chunks = np.concatenate([a.chunks for a, _ in entries])
         k = 0
         for si, (a, _) in enumerate(entries):
@@ -434,11 +464,12 @@ class RolloutServer:
                               self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
                               self.block_list.data_ptr(), self.block_count.data_ptr())
         st = _lib.stream_ptr()
-        if kind == "F":
-            _lib.check(self.lib.ddm_substep_forward(ctypes.byref(batch), addr[2], addr[3], st), "ddm_substep_forward")
+        if kind == "F":  # scenes without a tape in their record run a plain forward substep
+            _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(batch), addr[2], addr[3], st),
+                       "ddm_substep_forward_taped")
         else:
-            _lib.check(self.lib.ddm_substep_backward(ctypes.byref(batch), addr[2], addr[3], addr[4], addr[5], st),
-                       "ddm_substep_backward")
+            _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(batch), addr[2], addr[3], addr[4], addr[5],
+                                                           st), "ddm_substep_backward_taped")
         for a, _ in entries:
             self.particle_substeps += a.req.n if kind == "B" else 0
 
@@ -462,8 +493,8 @@ class RolloutServer:
         if fwd:
             self._launch(fwd, "F")
         if bwd:
-            for a, op in bwd:
-                a.g[op[3]].zero_()  # the adjoint substep adds into the state-t gradient
+            # the adjoint substep adds into the state-t gradient: clear all of them at once
+            torch._foreach_zero_([a.g[op[3]].flat for a, op in bwd])
             self._launch(bwd, "B")
         self.host_s["launch"] += time.perf_counter() - t0
         for a, _ in fwd + bwd:

@@ -122,7 +122,17 @@ typedef struct ddm_scene {
   const float *body_prim;      /* [n_bodies][4] (type, friction, softness, round) */
   const float *body_args;      /* [n_bodies][4] */
   float *tmp_Ft, *tmp_U, *tmp_V, *tmp_sig; /* [p_count][9],[..][9],[..][9],[..][3]; backward only */
-} ddm_scene;                   /* 136 bytes */
+  /* Grid snapshot of this scene for one substep, used only by the taped entry points
+   * (no counterpart in the reference, whose backward always recomputes P2G and the grid
+   * update, simulator.py:573-585). NULL tape_mmv = no snapshot. */
+  float *tape_mmv;             /* [tape_capacity][64][4] (m, mv) of the scene's listed blocks */
+  float *tape_v;               /* [tape_capacity][64][4] node velocity after the grid update */
+  int32_t *tape_list;          /* [tape_capacity] block indices */
+  int32_t *tape_status;        /* [2]: [0] blocks the scene listed (taped forward); [1] 1 if the
+                                  taped backward had to recompute the scene, else 0 */
+  int32_t tape_capacity;       /* blocks the snapshot holds */
+  int32_t reserved;
+} ddm_scene;                   /* 176 bytes */
 
 typedef struct ddm_batch {
   int32_t n_scenes;
@@ -172,45 +182,31 @@ int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur, const d
                                const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
                                hipEvent_t *events);
 
-/* Grid snapshot of one substep for the taped adjoint. No counterpart in the reference:
- * its backward always recomputes compute_svd, p2g and grid_op_v2 (simulator.py:573-585).
- * For callers with the memory to keep one snapshot (plus the F_tmp/U/V/sigma temporaries,
- * 120 B per particle) per substep of a rollout segment, so that the adjoint substep skips
- * that recompute. Device pointers; the struct itself is passed from the host. */
-typedef struct ddm_grid_tape {
-  float *grid_mmv;             /* [capacity][64][4] (m, mv) of the listed blocks, list order */
-  float *grid_v;               /* [capacity][64][4] node velocity after the grid update */
-  int32_t *block_list;         /* [capacity][2] (block, scene index) */
-  int32_t *status;             /* [2]: [0] blocks the substep listed (written by the forward);
-                                  [1] 1 if the backward had to recompute (more blocks than
-                                  capacity), else 0 (written by the backward) */
-  int32_t capacity;            /* blocks the snapshot holds */
-  int32_t reserved;
-} ddm_grid_tape;               /* 40 bytes */
-
-/* Forward substep that also records the tape: the temporaries go to the scenes'
- * tmp_Ft/tmp_U/tmp_V/tmp_sig (give each taped substep its own scene table) and the grid
- * of the listed blocks to *tape. Same results as ddm_substep_forward. */
+/* Taped substeps: the forward keeps what its adjoint needs, per scene, in the scene
+ * records (give each taped substep its own scene table): the SVD temporaries in
+ * tmp_Ft/tmp_U/tmp_V/tmp_sig and a snapshot of the scene's listed grid blocks in tape_*.
+ * No counterpart in the reference; for callers with the memory to keep them (120 B per
+ * particle plus 2 KiB per active block per substep). Scenes without a snapshot (NULL
+ * tape_mmv) run as in ddm_substep_forward. */
 int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                              const ddm_grid_tape *tape, hipStream_t stream);
+                              hipStream_t stream);
 
-/* Adjoint substep from a tape recorded by ddm_substep_forward_taped on the same states,
- * poses and scene table: restores the grid instead of recomputing P2G and the grid update.
- * When the forward listed more blocks than the tape holds, the kernels recompute them
- * (decided on the device, no host synchronisation) and set tape->status[1]. Same results
- * as ddm_substep_backward up to FP32 accumulation order. */
+/* Adjoint substep from the snapshots recorded by ddm_substep_forward_taped with the same
+ * scene table, states and poses: the grid is restored instead of recomputed. A scene
+ * that listed more blocks than its snapshot holds (or has none) is recomputed instead,
+ * decided on the device (no host synchronisation); its tape_status[1] is then 1. Same
+ * results as ddm_substep_backward up to FP32 accumulation order. */
 int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                               const ddm_state *cur_grad, const ddm_state *next_grad,
-                               const ddm_grid_tape *tape, hipStream_t stream);
+                               const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream);
 
 /* The taped substeps with hipEvents recorded between their stages (as the _timed
  * variants above): forward events[0..3] around P2G, grid update, G2P; backward events[0..4]
- * around restore (+ recompute when needed), G2P adjoint, grid adjoint, P2G adjoint. */
+ * around restore (+ recompute where needed), G2P adjoint, grid adjoint, P2G adjoint. */
 int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                    const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events);
+                                    hipStream_t stream, hipEvent_t *events);
 int ddm_substep_backward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                     const ddm_state *cur_grad, const ddm_state *next_grad,
-                                     const ddm_grid_tape *tape, hipStream_t stream, hipEvent_t *events);
+                                     const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
+                                     hipEvent_t *events);
 
 /* Zeroes the batch's grid arenas and block flags (after an error or on setup). */
 int ddm_batch_reset(const ddm_batch *b, hipStream_t stream);
