@@ -97,7 +97,7 @@ def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False):
     return eng, ro, gx
 
 
-def serve_benchmark(dev, rates, seconds, seed):
+def serve_benchmark(dev, rates, seconds, seed, graphs=True):
     """Served rollouts: for each Poisson arrival rate, a stream of uneven requests
     (particles U[5k,100k], grid 48^3/64^3/96^3, horizon U[20,200] with checkpoints every
     10 substeps, forward + backward each) through the continuous-batching server.
@@ -112,14 +112,15 @@ def serve_benchmark(dev, rates, seconds, seed):
         stream = W.served_stream(rate, n_requests, seed=seed + 1000 * k)
         reqs = [Request(i, d["scene"].spec, d["scene"].x0, d["pose_pos"], d["pose_rot"], d["target"],
                         arrival=d["arrival"], checkpoint_every=d["checkpoint_every"]) for i, d in enumerate(stream)]
-        srv = RolloutServer(device=dev)
+        srv = RolloutServer(device=dev, use_graphs=graphs)
         t0 = time.perf_counter()
         res = srv.run(reqs)
         wall = time.perf_counter() - t0
         st = latency_stats(res)
         ps = sum(r.n * r.T for r in reqs)
         out.append({"rate_req_s": rate, "requests": n_requests, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                    "throughput": ps / wall, "wall_s": wall, "ticks": srv.ticks,
+                    "throughput": ps / wall, "wall_s": wall, "ticks": srv.ticks, "graph_replays": srv.replays,
+                    "graphs_captured": len(srv.graphs),
                     "host_s": {k: round(v, 4) for k, v in srv.host_s.items()}})
         log(f"serve rate {rate}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
             f"throughput {ps / wall:.3e} particle-substeps/s")
@@ -163,6 +164,10 @@ class KernelTimer:
         taped = ro.tape is not None
         fwd_names, bwd_names = (FWD_NAMES_TAPED, BWD_NAMES_TAPED) if taped else (FWD_NAMES, BWD_NAMES)
         batches = [eng._batch_with_scenes(ro.tape.scene_tabs[t]) for t in range(ro.T)] if taped else None
+        # the adjoint substeps write the state-t gradients (as Rollout.backward does)
+        ow = _lib.DDM_GRAD_OVERWRITE
+        bwd_batches = ([eng._batch_with(ro.tape.scene_tabs[t], ow) for t in range(ro.T)] if taped else
+                       [eng._batch_with(None, ow)] * ro.T)
         for t in range(ro.T):
             evs, arr = events(4)
             if taped:
@@ -182,15 +187,14 @@ class KernelTimer:
         cur = 0
         for t in range(ro.T - 1, -1, -1):
             nxt, cur = cur, 1 - cur
-            ro.g[cur].zero_()
             tabs = (ro._sview(t, t).data_ptr(), ro._sview(t + 1, t + 1).data_ptr(), ro._gview(cur, t).data_ptr(),
                     ro._gview(nxt, t + 1).data_ptr())
             if taped:
                 evs, arr = events(5)
-                _lib.check(lib.ddm_substep_backward_taped_timed(ctypes.byref(batches[t]), *tabs, st, arr))
+                _lib.check(lib.ddm_substep_backward_taped_timed(ctypes.byref(bwd_batches[t]), *tabs, st, arr))
             else:
                 evs, arr = events(6)
-                _lib.check(lib.ddm_substep_backward_timed(ctypes.byref(eng._batch), *tabs, st, arr))
+                _lib.check(lib.ddm_substep_backward_timed(ctypes.byref(bwd_batches[t]), *tabs, st, arr))
             bwd_ev.append((evs, arr))
         torch.cuda.synchronize()
         tot = {k: 0.0 for k in fwd_names + bwd_names}
@@ -252,6 +256,7 @@ def main():
     ap.add_argument("--no-recompute", action="store_true",
                     help="skip the extra measurement of the recompute adjoint")
     ap.add_argument("--serve-rates", default="10,50,100,200", help="request arrival rates (req/s) to sweep")
+    ap.add_argument("--serve-eager", action="store_true", help="serve without capturing ticks in HIP graphs")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival window per rate: rate x seconds requests (at least 24)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -358,7 +363,8 @@ def main():
 
     serve = None
     if not args.no_serve:
-        serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_seconds, args.seed)
+        serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_seconds, args.seed,
+                                not args.serve_eager)
 
     cpu = None
     if not args.no_cpu:

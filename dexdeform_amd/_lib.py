@@ -58,8 +58,11 @@ class DdmBatch(ctypes.Structure):
         ("n_scenes", c_int32), ("n_chunks", c_int32), ("n_blocks", c_int32), ("max_bodies", c_int32),
         ("scenes", c_void_p), ("chunks", c_void_p), ("grid_mmv", c_void_p), ("grid_v", c_void_p),
         ("grid_dv", c_void_p), ("grid_dmmv", c_void_p), ("block_flags", c_void_p), ("block_list", c_void_p),
-        ("block_count", c_void_p),
+        ("block_count", c_void_p), ("live", c_void_p), ("options", c_int32), ("reserved", c_int32),
     ]
+
+
+DDM_GRAD_OVERWRITE = 1
 
 
 class DdmState(ctypes.Structure):

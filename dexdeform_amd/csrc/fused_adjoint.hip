@@ -119,6 +119,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   __shared__ Groups gr;
   __shared__ int s_box[6];
   float *const tx = sh.tile[0], *const ty = sh.tile[1], *const tz = sh.tile[2];
+  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
@@ -137,7 +138,8 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
     f3 gxp = g2p_particle_adj(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
                               ld3(next_grad.x + 3 * (size_t)p), ld3(next_grad.v + 3 * (size_t)p), gC, gv, none,
                               S.inv_dx, S.dx, S.dt, S.ground_height, dim, &g_nv, &st);
-    add3(cur_grad.x + 3 * (size_t)p, gxp);
+    if (B.options & DDM_GRAD_OVERWRITE) st3(cur_grad.x + 3 * (size_t)p, gxp);
+    else add3(cur_grad.x + 3 * (size_t)p, gxp);
     key = pack_base(st.base);
   } else {
     st.base[0] = st.base[1] = st.base[2] = 0;
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
+  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
@@ -223,12 +226,15 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
                    ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
   m33 gFt, gF, gC;
   svd_adj(F, C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
-  add3(cur_grad.x + p3, a.x);
-  add3(cur_grad.v + p3, a.v);
-  m33 cg = ld9(cur_grad.C + p9);
+  const bool overwrite = B.options & DDM_GRAD_OVERWRITE;
+  add3(cur_grad.x + p3, a.x);  // onto the G2P adjoint's contribution of this substep
+  if (overwrite) st3(cur_grad.v + p3, a.v);
+  else add3(cur_grad.v + p3, a.v);
+  m33 cg = overwrite ? m33{} : ld9(cur_grad.C + p9);
   cg = madd(madd(cg, a.C), gC);  // p2g_grad adds first, then compute_svd_grad (simulator.py:584-585)
   st9(cur_grad.C + p9, cg);
-  add9(cur_grad.F + p9, gF);
+  if (overwrite) st9(cur_grad.F + p9, gF);
+  else add9(cur_grad.F + p9, gF);
 }
 
 }  // namespace

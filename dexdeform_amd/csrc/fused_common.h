@@ -362,6 +362,10 @@ inline int grid_wgs(const ddm_batch *b) {
   return w < 4096 ? (w > 0 ? w : 1) : 4096;
 }
 
+// live chunks / scenes of a launch (a captured launch of fixed size reads them on the device)
+__device__ __forceinline__ int live_chunks(const ddm_batch &B) { return B.live ? B.live[0] : B.n_chunks; }
+__device__ __forceinline__ int live_scenes(const ddm_batch &B) { return B.live ? B.live[1] : B.n_scenes; }
+
 }  // namespace
 
 // Adjoint kernels (fused_adjoint.hip, compiled with FMA contraction: they feed only

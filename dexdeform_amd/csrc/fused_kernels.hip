@@ -22,6 +22,7 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
   __shared__ Groups gr;
   __shared__ int s_box[6];
   float *const tm = sh.tile[0], *const tx = sh.tile[1], *const ty = sh.tile[2], *const tz = sh.tile[3];
+  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   if (MODE == kRecomputeUntaped && S.tape_mmv && S.tape_status[1] == 0) return;  // uniform: one scene per chunk
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
 // Taped forward, first: empties every snapshot of the launch (scenes without particles
 // included, so that no stale snapshot survives).
 __global__ __launch_bounds__(kWG) void k_tape_reset(ddm_batch B) {
-  for (int si = threadIdx.x; si < B.n_scenes; si += kWG)
+  for (int si = threadIdx.x; si < live_scenes(B); si += kWG)
     if (B.scenes[si].tape_mmv) B.scenes[si].tape_status[0] = 0;
 }
 
@@ -196,14 +197,15 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // one row of workgroups per scene.
 constexpr int kRestoreWGs = 16;
 __global__ __launch_bounds__(kWG) void k_restore(ddm_batch B) {
-  const int si = (int)blockIdx.y;
+  const int si = (int)blockIdx.y, n_live = live_scenes(B);
+  if (si >= n_live) return;
   const ddm_scene &S = B.scenes[si];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // restored entries of the scenes before this one, and of all scenes
   int before = 0, total = 0;
-  for (int s0 = 0; s0 < B.n_scenes; s0 += 64) {
+  for (int s0 = 0; s0 < n_live; s0 += 64) {
     const int s2 = s0 + lane;
-    const int c = s2 < B.n_scenes ? restorable(B.scenes[s2]) : 0;
+    const int c = s2 < n_live ? restorable(B.scenes[s2]) : 0;
     total += wave_sum_i(c);
     before += wave_sum_i(s2 < si ? c : 0);
   }
@@ -227,6 +229,7 @@ __global__ __launch_bounds__(kWG) void k_restore(ddm_batch B) {
 __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                              const ddm_state *__restrict__ next_tab) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
+  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];

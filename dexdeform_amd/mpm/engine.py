@@ -277,17 +277,21 @@ class Engine:
         with x, v, F, C, body_pos, body_rot tensors laid out like them)."""
         self.forward_tables(self.table_of(cur), self.table_of(nxt), stream)
 
-    def backward(self, cur, nxt, gcur, gnxt, stream=None):
-        self.backward_tables(self.table_of(cur), self.table_of(nxt), self.table_of(gcur), self.table_of(gnxt), stream)
+    def backward(self, cur, nxt, gcur, gnxt, stream=None, overwrite=False):
+        self.backward_tables(self.table_of(cur), self.table_of(nxt), self.table_of(gcur), self.table_of(gnxt), stream,
+                             overwrite)
 
     def forward_tables(self, cur_tab, nxt_tab, stream=None):
         _lib.check(self.lib.ddm_substep_forward(ctypes.byref(self._batch), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                 _lib.stream_ptr(stream)), "ddm_substep_forward")
 
-    def backward_tables(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, stream=None):
+    def backward_tables(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, stream=None, overwrite=False):
+        """Adjoint substep; adds into the current state's gradients, or writes them
+        (overwrite=True: no clearing needed beforehand)."""
         if not self.with_grad:
             raise RuntimeError("engine was created with with_grad=False")
-        _lib.check(self.lib.ddm_substep_backward(ctypes.byref(self._batch), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+        b = self._batch_with(options=_lib.DDM_GRAD_OVERWRITE if overwrite else 0)
+        _lib.check(self.lib.ddm_substep_backward(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                  gcur_tab.data_ptr(), gnxt_tab.data_ptr(), _lib.stream_ptr(stream)),
                    "ddm_substep_backward")
 
@@ -326,10 +330,15 @@ class Engine:
             off += int(caps[i])
         return torch.as_tensor(tab.view(np.uint8).copy(), device=self.device)
 
-    def _batch_with_scenes(self, scene_tab):
+    def _batch_with(self, scene_tab=None, options=0):
         b = _lib.DdmBatch.from_buffer_copy(self._batch)
-        b.scenes = scene_tab.data_ptr()
+        if scene_tab is not None:
+            b.scenes = scene_tab.data_ptr()
+        b.options = options
         return b
+
+    def _batch_with_scenes(self, scene_tab):
+        return self._batch_with(scene_tab)
 
     def forward_taped(self, cur_tab, nxt_tab, scene_tab, stream=None):
         """Forward substep recording, per scene, the snapshot and temporaries that the
@@ -338,11 +347,11 @@ class Engine:
         _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                       _lib.stream_ptr(stream)), "ddm_substep_forward_taped")
 
-    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, stream=None):
+    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, stream=None, overwrite=False):
         """Adjoint substep from what forward_taped recorded with the same scene table."""
         if not self.with_grad:
             raise RuntimeError("engine was created with with_grad=False")
-        b = self._batch_with_scenes(scene_tab)
+        b = self._batch_with(scene_tab, _lib.DDM_GRAD_OVERWRITE if overwrite else 0)
         _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                        gcur_tab.data_ptr(), gnxt_tab.data_ptr(),
                                                        _lib.stream_ptr(stream)), "ddm_substep_backward_taped")

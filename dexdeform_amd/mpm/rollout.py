@@ -196,13 +196,13 @@ class Rollout:
                 t = base + j
                 nxt_slot = cur_slot
                 cur_slot = 1 - cur_slot
-                self.g[cur_slot].zero_()
                 tabs = (self._sview(j, t), self._sview(j + 1, t + 1), self._gview(cur_slot, t),
                         self._gview(nxt_slot, t + 1))
+                # the adjoint substep writes (not adds to) the state-t gradient: no clearing
                 if self.tape is not None:
-                    eng.backward_taped(*tabs, self.tape.scene_tabs[j])
+                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], overwrite=True)
                 else:
-                    eng.backward_tables(*tabs)
+                    eng.backward_tables(*tabs, overwrite=True)
         self._g0 = self.g[cur_slot]
         return self._g0
 

@@ -16,9 +16,17 @@ Requests of different particle count, grid size and horizon share the GPU:
   substeps run in one fused adjoint launch sequence (``ddm_substep_forward`` /
   ``ddm_substep_backward`` with per-scene state tables).
 * state stays on the device for the whole request; only the small scene/chunk/state
-  tables are uploaded per launch, from a pinned host ring with asynchronous copies, so
-  the host prepares the next tick while the GPU runs the current one (a synchronous
-  pageable upload would wait for all queued kernels first).
+  tables are uploaded per tick, from a pinned host ring with one asynchronous copy into
+  tables at fixed device addresses, so the host prepares the next tick while the GPU
+  runs the current one (a synchronous pageable upload would wait for all queued kernels
+  first).
+* graphs: a tick's two launch sequences are captured once per launch size (chunk counts
+  rounded up to powers of two) into a HIP graph and replayed on later ticks of that
+  size; the kernels read the live chunk and scene counts from the uploaded tables and
+  the chunks beyond them exit at once. ``use_graphs=False`` launches every tick eagerly,
+  sized exactly.
+* gradients: the adjoint launches write (not add to) the state-t gradients
+  (DDM_GRAD_OVERWRITE), so no clearing kernels run between ticks.
 
 Latency of a request = completion (an event recorded after its last operation) minus
 arrival; throughput = particle-substeps (forward + adjoint) per second of wall time.
@@ -143,50 +151,77 @@ def _to_device(a, device, dtype=np.float32) -> torch.Tensor:
     return h.to(device, non_blocking=True)
 
 
-class _Staging:
-    """Per-launch tables: packed into one slot of a pinned host ring, copied
-    asynchronously into one device buffer on the launch stream. Launches on the same
-    stream consume the device buffer in order, so one device buffer suffices; a host
-    slot is reused only after the copy that read it has completed."""
+class _TickTables:
+    """Per-tick launch tables at fixed device addresses, so that a tick's launches can be
+    captured once in a HIP graph and replayed with each tick's contents.
+
+    Layout (parts 256-byte aligned): for the forward launch and then the adjoint launch,
+    the scene records [max_scenes], the state records (current, next; for the adjoint
+    also their gradients) [max_scenes] and the live counts (chunks, scenes); then each
+    launch's chunk table [max_chunks]. A tick's contents are packed into one slot of a
+    pinned host ring and copied to the device in one asynchronous transfer on the
+    launch stream; a host slot is reused only after the copy that read it completed.
+    Launches on the same stream consume the device tables in order, so one device copy
+    suffices."""
 
     ALIGN = 256
+    PARTS = {"F": ("recs", "cur", "nxt", "live"), "B": ("recs", "cur", "nxt", "gcur", "gnxt", "live")}
 
-    def __init__(self, device, nbytes=1 << 20, slots=8):
+    def __init__(self, device, max_scenes, max_chunks, slots=8):
         self.device = device
-        self.slots = slots
-        self._alloc(nbytes)
-
-    def _alloc(self, nbytes):
-        self.nbytes = int(nbytes)
-        self.dev = torch.empty(self.nbytes, dtype=torch.uint8, device=self.device)
-        self.host = [torch.empty(self.nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(self.slots)]
-        self.events = [None] * self.slots
+        self.max_scenes = int(max_scenes)
+        self.max_chunks = int(max_chunks)
+        a = lambda n: (n + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        size = {"recs": _lib.SCENE_DTYPE.itemsize * self.max_scenes, "cur": 48 * self.max_scenes,
+                "nxt": 48 * self.max_scenes, "gcur": 48 * self.max_scenes, "gnxt": 48 * self.max_scenes,
+                "live": 8, "chunks": 16 * self.max_chunks}
+        self.off, off = {}, 0
+        for kind in ("F", "B"):
+            for part in self.PARTS[kind]:
+                self.off[kind, part] = off
+                off += a(size[part])
+        for kind in ("F", "B"):
+            self.off[kind, "chunks"] = off
+            off += a(size["chunks"])
+        self.size = size
+        self.nbytes = off
+        self.dev = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.host = [torch.empty(self.nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+        self.events = [None] * slots
         self.i = 0
+        base = self.dev.data_ptr()
+        self.addr = {k: base + o for k, o in self.off.items()}
 
-    def upload(self, parts):
-        """parts: numpy arrays; returns their device addresses (valid until the next
-        upload's copy runs, i.e. for the launch that follows on the same stream)."""
-        sizes = [int(a.nbytes) for a in parts]
-        need = sum((n + self.ALIGN - 1) // self.ALIGN * self.ALIGN for n in sizes)
-        if need > self.nbytes:
-            torch.cuda.synchronize()
-            self._alloc(max(need, 2 * self.nbytes))
+    def fits(self, n_scenes, n_chunks):
+        return n_scenes <= self.max_scenes and n_chunks <= self.max_chunks
+
+    def stage(self, contents):
+        """contents: {(kind, part): numpy array}; packed at the fixed offsets and copied."""
         slot = self.i
-        self.i = (self.i + 1) % self.slots
+        self.i = (self.i + 1) % len(self.host)
         if self.events[slot] is not None:
             self.events[slot].synchronize()
         h = self.host[slot].numpy()
-        base = self.dev.data_ptr()
-        off, addrs = 0, []
-        for a, n in zip(parts, sizes):
-            h[off:off + n] = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
-            addrs.append(base + off)
-            off += (n + self.ALIGN - 1) // self.ALIGN * self.ALIGN
-        self.dev[:off].copy_(self.host[slot][:off], non_blocking=True)
+        end = 0
+        for key, arr in contents.items():
+            b = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+            o = self.off[key]
+            if b.size > self.size[key[1]]:
+                raise ValueError(f"{key} table exceeds its capacity")
+            h[o:o + b.size] = b
+            end = max(end, o + b.size)
+        self.dev[:end].copy_(self.host[slot][:end], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record()
         self.events[slot] = ev
-        return addrs
+
+
+def _bucket(n: int, least: int) -> int:
+    """Launch capacity of a captured tick: the power of two >= n (at least `least`)."""
+    b = least
+    while b < n:
+        b *= 2
+    return b
 
 
 def _state_records(records) -> np.ndarray:
@@ -387,7 +422,7 @@ class _Active:
 
 class RolloutServer:
     def __init__(self, device="cuda:0", arena_blocks: int = 64 * 13824, memory_budget: int = 64 << 30,
-                 max_active: int = 256):
+                 max_active: int = 256, use_graphs: bool = True):
         self.lib = _lib.load()
         self.device = torch.device(device)
         self.arena = BlockArena(arena_blocks)
@@ -403,11 +438,16 @@ class RolloutServer:
         self.active: List[_Active] = []
         self.results: Dict[int, Result] = {}
         self.used_bytes = 0
-        self.staging = _Staging(self.device)
+        # per-tick tables; chunk capacity grows (and captured graphs are dropped) on demand
+        self.tables = _TickTables(self.device, max_active, 4096)
+        self.use_graphs = use_graphs
+        self.capture_stream = torch.cuda.Stream(self.device)
+        self.graphs: Dict[tuple, torch.cuda.CUDAGraph] = {}
         self.host_s = {"admit": 0.0, "launch": 0.0, "tick": 0.0}  # host seconds spent per phase
         self.t0 = None
         self.ticks = 0
         self.particle_substeps = 0
+        self.replays = 0  # ticks that replayed a captured graph
 
     # ------------------------------------------------------------------ admission
     def submit(self, req: Request):
@@ -442,36 +482,86 @@ class RolloutServer:
             self.host_s["admit"] += time.perf_counter() - t0
 
     # ------------------------------------------------------------------ launches
-    def _launch(self, entries, kind):
-        """entries: list of (active, op). One fused launch sequence for all of them."""
+    def _contents(self, entries, kind):
+        """Tables of one fused launch over entries [(active, op)]: scene records, chunk
+        table (column 0 = position of the scene in the launch), state records."""
         recs = np.concatenate([a.record_for(op) for a, op in entries])
         chunks = np.concatenate([a.chunks for a, _ in entries])
         k = 0
         for si, (a, _) in enumerate(entries):
             chunks[k:k + len(a.chunks), 0] = si
             k += len(a.chunks)
-        cur = [a.state_record(op[1], op[2]) for a, op in entries]
-        nxt = [a.state_record(op[1] + 1, op[2] + 1) for a, op in entries]
-        parts = [recs, chunks, _state_records(cur), _state_records(nxt)]
+        c = {(kind, "recs"): recs, (kind, "chunks"): chunks,
+             (kind, "cur"): _state_records([a.state_record(op[1], op[2]) for a, op in entries]),
+             (kind, "nxt"): _state_records([a.state_record(op[1] + 1, op[2] + 1) for a, op in entries]),
+             (kind, "live"): np.array([len(chunks), len(entries)], dtype=np.int32)}
         if kind == "B":
-            parts += [_state_records([a.grad_record(op[3], op[2]) for a, op in entries]),
-                      _state_records([a.grad_record(op[4], op[2] + 1) for a, op in entries])]
-        addr = self.staging.upload(parts)
-        # blocks addressable by this launch: the grid kernels size their launch by it
-        n_blocks = max(a.blk_begin + a.n_blocks for a, _ in entries)
-        batch = _lib.DdmBatch(len(entries), len(chunks), n_blocks, max(a.req.spec.n_bodies for a, _ in entries),
-                              addr[0], addr[1], self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
+            c[kind, "gcur"] = _state_records([a.grad_record(op[3], op[2]) for a, op in entries])
+            c[kind, "gnxt"] = _state_records([a.grad_record(op[4], op[2] + 1) for a, op in entries])
+        return c
+
+    def _launch(self, kind, n_scenes, n_chunks, live):
+        """One fused launch sequence over the staged tables of `kind` ("F": forward-type
+        substeps, "B": adjoint substeps). With `live`, the launch is sized for n_scenes /
+        n_chunks and its kernels read the counts actually used from the staged tables
+        (what a captured tick replays); without, it is sized exactly."""
+        A = self.tables.addr
+        batch = _lib.DdmBatch(n_scenes, n_chunks, self.arena.n, _lib.DDM_MAX_BODIES, A[kind, "recs"],
+                              A[kind, "chunks"], self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
                               self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
-                              self.block_list.data_ptr(), self.block_count.data_ptr())
+                              self.block_list.data_ptr(), self.block_count.data_ptr(),
+                              A[kind, "live"] if live else None, _lib.DDM_GRAD_OVERWRITE)
         st = _lib.stream_ptr()
         if kind == "F":  # scenes without a tape in their record run a plain forward substep
-            _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(batch), addr[2], addr[3], st),
+            _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(batch), A["F", "cur"], A["F", "nxt"], st),
                        "ddm_substep_forward_taped")
-        else:
-            _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(batch), addr[2], addr[3], addr[4], addr[5],
-                                                           st), "ddm_substep_backward_taped")
-        for a, _ in entries:
-            self.particle_substeps += a.req.n if kind == "B" else 0
+        else:  # the adjoint substep writes the state-t gradients: nothing to clear first
+            _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(batch), A["B", "cur"], A["B", "nxt"],
+                                                           A["B", "gcur"], A["B", "gnxt"], st),
+                       "ddm_substep_backward_taped")
+
+    def _launch_tick(self, key, live):
+        """key: (forward scenes, forward chunks, adjoint scenes, adjoint chunks); 0 = no launch."""
+        fs, fc, bs, bc = key
+        if fs:
+            self._launch("F", fs, fc, live)
+        if bs:
+            self._launch("B", bs, bc, live)
+
+    def _run_tick(self, fwd, bwd):
+        contents = {}
+        if fwd:
+            contents.update(self._contents(fwd, "F"))
+        if bwd:
+            contents.update(self._contents(bwd, "B"))
+        n_chunks = max(len(contents[k, "chunks"]) if (k, "chunks") in contents else 0 for k in "FB")
+        if not self.tables.fits(max(len(fwd), len(bwd)), n_chunks):
+            torch.cuda.synchronize()  # the old tables may still be read by queued launches
+            self.tables = _TickTables(self.device, max(self.tables.max_scenes, len(fwd), len(bwd)),
+                                      _bucket(n_chunks, self.tables.max_chunks))
+            self.graphs.clear()
+        self.tables.stage(contents)
+        if not self.use_graphs:
+            exact = (len(fwd), len(contents.get(("F", "chunks"), ())), len(bwd), len(contents.get(("B", "chunks"), ())))
+            self._launch_tick(exact, live=False)
+            return
+        # launch sizes of the captured tick: every scene slot of the tables, and the chunk
+        # count rounded up to a power of two (one graph per pair of chunk buckets)
+        cap_s, cap_c = self.tables.max_scenes, self.tables.max_chunks
+        key = (cap_s if fwd else 0, min(_bucket(len(contents["F", "chunks"]), 64), cap_c) if fwd else 0,
+               cap_s if bwd else 0, min(_bucket(len(contents["B", "chunks"]), 64), cap_c) if bwd else 0)
+        g = self.graphs.get(key)
+        if g is None:
+            # first tick of this shape: capture its launches on a side stream (capture runs
+            # nothing, and the launches allocate nothing, so queued work need not drain)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(self.capture_stream):
+                g.capture_begin(capture_error_mode="thread_local")
+                self._launch_tick(key, live=True)
+                g.capture_end()
+            self.graphs[key] = g
+        g.replay()
+        self.replays += 1
 
     def tick(self):
         t_tick = time.perf_counter()
@@ -490,13 +580,11 @@ class RolloutServer:
                 continue
             (fwd if op[0] == "F" else bwd).append((a, op))
         t0 = time.perf_counter()
-        if fwd:
-            self._launch(fwd, "F")
-        if bwd:
-            # the adjoint substep adds into the state-t gradient: clear all of them at once
-            torch._foreach_zero_([a.g[op[3]].flat for a, op in bwd])
-            self._launch(bwd, "B")
+        if fwd or bwd:
+            self._run_tick(fwd, bwd)
         self.host_s["launch"] += time.perf_counter() - t0
+        for a, _ in bwd:
+            self.particle_substeps += a.req.n
         for a, _ in fwd + bwd:
             a.pc += 1
         if fwd or bwd:

@@ -107,6 +107,7 @@ int particle2mass(float *particle_x, float *particle_m, int *grid_lower, const i
  * n_blocks consecutive blocks starting at blk_begin. */
 
 #define DDM_MAX_BODIES 64
+#define DDM_GRAD_OVERWRITE 1  /* ddm_batch.options */
 
 typedef struct ddm_scene {
   int32_t p_count;             /* particles of the scene (chunks define the work) */
@@ -151,6 +152,12 @@ typedef struct ddm_batch {
   int32_t *block_list;         /* [n_blocks][2] (block, scene index) of the blocks P2G flagged, in
                                   flagging order; the grid kernels visit exactly these */
   int32_t *block_count;        /* [1] entries in block_list; zero between substeps */
+  const int32_t *live;         /* optional device [2] = (chunks, scenes) actually used, <= n_chunks
+                                  and n_scenes: lets one captured launch of fixed size serve
+                                  batches of any size (NULL = n_chunks, n_scenes) */
+  int32_t options;             /* DDM_GRAD_OVERWRITE: the adjoint substep writes the current
+                                  state's x/v/F/C gradients instead of adding to them */
+  int32_t reserved;
 } ddm_batch;
 
 /* Per-scene state record (device memory, one per scene, in scene order). */

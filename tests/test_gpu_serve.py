@@ -130,6 +130,31 @@ def test_server_results_alone_equal_mixed():
         assert el <= 1e-4 and ex <= 1e-4 and eg <= 1e-3 and e0 <= 1e-3
 
 
+def test_server_graph_ticks_match_eager_ticks():
+    """Ticks replayed from captured HIP graphs (launches sized to power-of-two buckets,
+    live counts read on the device) give the results of exactly sized eager launches."""
+    results, servers = {}, {}
+    for graphs in (True, False):
+        reqs = _requests()
+        for k, r in enumerate(reqs):
+            r.arrival = 0.002 * k
+        srv = RolloutServer(device=_dev(), arena_blocks=16384, memory_budget=8 << 30, use_graphs=graphs)
+        results[graphs] = srv.run(reqs)
+        servers[graphs] = srv
+    g = servers[True]
+    print(f"graph server: {g.ticks} ticks, {g.replays} replayed, {len(g.graphs)} graphs captured")
+    assert g.replays == g.ticks, "every tick replays a captured graph"
+    assert len(g.graphs) < g.ticks // 4, "graphs are reused across ticks"
+    assert servers[False].replays == 0
+    for rid, a in results[False].items():
+        b = results[True][rid]
+        el = abs(a.loss - b.loss) / abs(a.loss)
+        eg = rel_err(b.pos_grad.cpu().numpy(), a.pos_grad.cpu().numpy())
+        e0 = rel_err(b.x0_grad.cpu().numpy(), a.x0_grad.cpu().numpy())
+        print(f"request {rid}: graph vs eager: loss {el:.1e} pose grad {eg:.1e} x0 grad {e0:.1e}")
+        assert el <= 1e-5 and eg <= 1e-3 and e0 <= 1e-3
+
+
 def test_server_matches_direct_rollout():
     r = _requests()[1]
     srv = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30)
