@@ -53,16 +53,30 @@ SCENE_DTYPE = np.dtype([
 assert SCENE_DTYPE.itemsize == ctypes.sizeof(DdmScene) == 176
 
 
+class DdmDet(ctypes.Structure):
+    """Workspace of the deterministic sorted-reduction mode (device arrays)."""
+    _fields_ = [
+        ("cell_count", c_void_p), ("cell_start", c_void_p), ("order", c_void_p), ("order_p", c_void_p),
+        ("key_slot", c_void_p), ("records", c_void_p), ("body_partial", c_void_p), ("range_count", c_void_p),
+        ("counter", c_void_p), ("reserved", c_int32 * 2),
+    ]
+
+
+assert ctypes.sizeof(DdmDet) == 80
+
+
 class DdmBatch(ctypes.Structure):
     _fields_ = [
         ("n_scenes", c_int32), ("n_chunks", c_int32), ("n_blocks", c_int32), ("max_bodies", c_int32),
         ("scenes", c_void_p), ("chunks", c_void_p), ("grid_mmv", c_void_p), ("grid_v", c_void_p),
         ("grid_dv", c_void_p), ("grid_dmmv", c_void_p), ("block_flags", c_void_p), ("block_list", c_void_p),
-        ("block_count", c_void_p), ("live", c_void_p), ("options", c_int32), ("reserved", c_int32),
+        ("block_count", c_void_p), ("live", c_void_p), ("det", ctypes.POINTER(DdmDet)), ("options", c_int32),
+        ("reserved", c_int32),
     ]
 
 
 DDM_GRAD_OVERWRITE = 1
+DDM_DETERMINISTIC = 2
 
 
 class DdmState(ctypes.Structure):

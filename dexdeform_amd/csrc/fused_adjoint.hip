@@ -6,8 +6,12 @@
 namespace ddm_fused {
 namespace {
 
-// adjoint of the grid update; clears (m, mv), the velocity adjoint and the flags
-__global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+// adjoint of the grid update; clears (m, mv), the velocity adjoint and the flags.
+// DET (deterministic mode): each block writes its body-pose gradient sums to its list
+// entry's partials (k_det_body_reduce adds them up in block order) instead of atomics,
+// and clears its bin counts.
+template <bool DET>
+__global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const ddm_state *__restrict__ cur_tab,
                                                   const ddm_state *__restrict__ next_tab,
                                                   const ddm_state *__restrict__ cur_grad_tab,
                                                   const ddm_state *__restrict__ next_grad_tab) {
@@ -70,7 +74,23 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *
           contact_adj(vin, gx, Bd, P.dt, mode, g, pg);
         }
       }
-      if (__any(mine)) {
+      if (DET) {
+        float4 *part = reinterpret_cast<float4 *>(D.body_partial + ((size_t)e * B.max_bodies + b) * 16);
+        if (__any(mine)) {
+          float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
+                            pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
+#pragma unroll
+          for (int q = 0; q < 14; ++q) vals[q] = wave_sum(vals[q]);
+          if (lane == 0) {
+            part[0] = make_float4(vals[0], vals[1], vals[2], vals[3]);
+            part[1] = make_float4(vals[4], vals[5], vals[6], vals[7]);
+            part[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
+            part[3] = make_float4(vals[12], vals[13], 0.f, 0.f);
+          }
+        } else if (lane < 4) {
+          part[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      } else if (__any(mine)) {
         float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
                           pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
 #pragma unroll
@@ -106,6 +126,57 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, const ddm_state *
     mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
     dv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane == 0) B.block_flags[blk] = 0u;
+    if (DET) D.cell_count[gi] = 0;  // the sorted gathers of this substep are done
+  }
+}
+
+// Deterministic mode: per scene (one wave), each body's pose gradients summed over the
+// scene's listed blocks in block order (the rebuilt list is sorted by block; a scene's
+// blocks are a contiguous range), then added once to the current and next substep's
+// body-pose gradients.
+__global__ __launch_bounds__(64) void k_det_body_reduce(ddm_batch B, ddm_det D,
+                                                        const ddm_state *__restrict__ cur_grad_tab,
+                                                        const ddm_state *__restrict__ next_grad_tab) {
+  const int si = (int)blockIdx.x;
+  if (si >= live_scenes(B)) return;
+  const ddm_scene &S = B.scenes[si];
+  const int lane = threadIdx.x;
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  auto lower = [&](int blk) {
+    int lo = 0, hi = n_listed;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (list[mid].x < blk) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int e0 = lower(S.blk_begin), e1 = lower(S.blk_begin + S.n_blocks);
+  if (e0 == e1) return;
+  const ddm_state cg = cur_grad_tab[si], ng = next_grad_tab[si];
+  for (int b = 0; b < S.n_bodies; ++b) {
+    float v[14];
+#pragma unroll
+    for (int q = 0; q < 14; ++q) v[q] = 0.f;
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const float4 *part = reinterpret_cast<const float4 *>(D.body_partial + ((size_t)e * B.max_bodies + b) * 16);
+      const float4 p0 = part[0], p1 = part[1], p2 = part[2], p3 = part[3];
+      v[0] += p0.x, v[1] += p0.y, v[2] += p0.z, v[3] += p0.w;
+      v[4] += p1.x, v[5] += p1.y, v[6] += p1.z, v[7] += p1.w;
+      v[8] += p2.x, v[9] += p2.y, v[10] += p2.z, v[11] += p2.w;
+      v[12] += p3.x, v[13] += p3.y;
+    }
+#pragma unroll
+    for (int q = 0; q < 14; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+      float *cp = cg.body_pos + 3 * b, *cr = cg.body_rot + 4 * b;
+      float *np = ng.body_pos + 3 * b, *nr = ng.body_rot + 4 * b;
+      cp[0] += v[0], cp[1] += v[1], cp[2] += v[2];
+      cr[0] += v[3], cr[1] += v[4], cr[2] += v[5], cr[3] += v[6];
+      np[0] += v[7], np[1] += v[8], np[2] += v[9];
+      nr[0] += v[10], nr[1] += v[11], nr[2] += v[12], nr[3] += v[13];
+    }
   }
 }
 
@@ -203,6 +274,69 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   }
 }
 
+// Deterministic G2P adjoint, per particle: the x adjoint as k_g2p_adj, and the record
+// (fx | node-velocity adjoint seed, C adjoint) in the particle's launch slot for the
+// sorted gather (the bins are the P2G pass's: same positions, same stencil bases).
+__global__ __launch_bounds__(kWG) void k_g2p_adj_det(ddm_batch B, ddm_det D, const ddm_state *__restrict__ cur_tab,
+                                                     const ddm_state *__restrict__ next_tab,
+                                                     const ddm_state *__restrict__ cur_grad_tab,
+                                                     const ddm_state *__restrict__ next_grad_tab) {
+  if ((int)blockIdx.x >= live_chunks(B)) return;
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int p = ch.y + (int)threadIdx.x;
+  if (p >= ch.z) return;
+  const ddm_scene &S = B.scenes[ch.x];
+  const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
+  const ddm_state cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
+  NullScatter none;
+  Stencil st;
+  f3 g_nv{0.f, 0.f, 0.f};
+  const m33 gC = ld9(next_grad.C + 9 * (size_t)p);
+  f3 gxp = g2p_particle_adj(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p), ld3(next_grad.x + 3 * (size_t)p),
+                            ld3(next_grad.v + 3 * (size_t)p), gC, gv, none, S.inv_dx, S.dx, S.dt, S.ground_height,
+                            dim, &g_nv, &st);
+  if (B.options & DDM_GRAD_OVERWRITE) st3(cur_grad.x + 3 * (size_t)p, gxp);
+  else add3(cur_grad.x + 3 * (size_t)p, gxp);
+  float4 *r = reinterpret_cast<float4 *>(D.records) + 4 * ((size_t)blockIdx.x * kWG + threadIdx.x);
+  r[0] = make_float4(st.fx[0], st.fx[1], st.fx[2], 0.f);
+  r[1] = make_float4(g_nv.x, g_nv.y, g_nv.z, gC.a[0]);
+  r[2] = make_float4(gC.a[1], gC.a[2], gC.a[3], gC.a[4]);
+  r[3] = make_float4(gC.a[5], gC.a[6], gC.a[7], gC.a[8]);
+}
+// one particle's node-velocity adjoint at stencil offset (I, J, K): k_g2p_adj's per-node operations
+struct G2PAdjDetContrib {
+  float inv_dx;
+  template <int I, int J, int K>
+  __device__ __forceinline__ void add(float4 a, float4 b, float4 c, float4 d, float *acc) const {
+    const float wxy = bspline_w<I>(a.x) * bspline_w<J>(a.y);
+    const float d0 = (float)I - a.x, d1 = (float)J - a.y;
+    const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
+    const float w = wxy * bspline_w<K>(a.z);
+    const float d2 = (float)K - a.z;
+    const float xx = w * inv_dx * 4.f;
+    acc[0] += w * b.x + (p0 + c.y * d2) * xx;
+    acc[1] += w * b.y + (p1 + d.x * d2) * xx;
+    acc[2] += w * b.z + (p2 + d.w * d2) * xx;
+  }
+};
+__global__ __launch_bounds__(kWG) void k_det_gather_g2p_adj(ddm_batch B, ddm_det D) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const int2 entry = list[e];
+    const ddm_scene &S = B.scenes[entry.y];
+    const BlockCoord c = block_node(S, entry.x, lane);
+    float acc[3] = {0.f, 0.f, 0.f};
+    G2PAdjDetContrib k{S.inv_dx};
+    det_gather_node<3>(S, D, c.ix, c.iy, c.iz, k, acc);
+    reinterpret_cast<float4 *>(B.grid_dv)[((size_t)entry.x << 6) | (size_t)lane] =
+        make_float4(acc[0], acc[1], acc[2], 0.f);
+  }
+}
+
 // adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
 // at most 168 VGPRs (3 waves per SIMD instead of 2, with a few bytes of spill): 11 % faster
 __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
@@ -245,7 +379,17 @@ void launch_g2p_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *n
 }
 void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                      const ddm_state *next_grad, hipStream_t stream) {
-  k_grid_adj<<<grid_wgs(&b), kWG, 0, stream>>>(b, cur, next, cur_grad, next_grad);
+  k_grid_adj<false><<<grid_wgs(&b), kWG, 0, stream>>>(b, ddm_det{}, cur, next, cur_grad, next_grad);
+}
+void launch_g2p_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
+                        const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
+  k_g2p_adj_det<<<b.n_chunks, kWG, 0, stream>>>(b, d, cur, next, cur_grad, next_grad);
+  k_det_gather_g2p_adj<<<grid_wgs(&b), kWG, 0, stream>>>(b, d);
+}
+void launch_grid_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
+                         const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
+  k_grid_adj<true><<<grid_wgs(&b), kWG, 0, stream>>>(b, d, cur, next, cur_grad, next_grad);
+  k_det_body_reduce<<<b.n_scenes, 64, 0, stream>>>(b, d, cur_grad, next_grad);
 }
 void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
                     hipStream_t stream) {

@@ -362,6 +362,77 @@ inline int grid_wgs(const ddm_batch *b) {
   return w < 4096 ? (w > 0 ? w : 1) : 4096;
 }
 
+// ---------------------------------------------------------------- deterministic mode
+// Sorted reductions instead of atomics (options bit DDM_DETERMINISTIC): particles are
+// binned by stencil-base node with a counting sort (ranks within a node sorted back into
+// particle-index order), and each grid node sums its contributions in a fixed order --
+// the 27 stencil offsets in (i, j, k) order, then the particles whose base is the node
+// minus that offset, in ascending particle index. The block list is rebuilt from the
+// flags in block order, which fixes the order of the body-pose reductions too.
+__device__ __forceinline__ bool deterministic(const ddm_batch &B) { return (B.options & DDM_DETERMINISTIC) != 0; }
+
+// B-spline weight of offset i for the fractional position f (stencil()'s expressions)
+template <int I>
+__device__ __forceinline__ float bspline_w(float f) {
+  if (I == 0) {
+    const float a = 1.5f - f;
+    return 0.5f * (a * a);
+  } else if (I == 1) {
+    const float a = f - 1.f;
+    return 0.75f - a * a;
+  } else {
+    const float a = f - 0.5f;
+    return 0.5f * (a * a);
+  }
+}
+
+// flag-only marking (the block list is rebuilt from the flags): the flag holds scene + 1
+__device__ void flag_tile_blocks_det(const ddm_batch &B, const ddm_scene &S, int si, const Tile &T) {
+  int b0[3], nb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b0[d] = T.lo[d] >> 2;
+    nb[d] = ((T.lo[d] + T.t[d] - 1) >> 2) - b0[d] + 1;
+  }
+  const int tot = nb[0] * nb[1] * nb[2];
+  for (int i = threadIdx.x; i < tot; i += kWG) {
+    const int bx = i / (nb[1] * nb[2]), by = (i / nb[2]) % nb[1], bz = i % nb[2];
+    B.block_flags[block_of(S, b0[0] + bx, b0[1] + by, b0[2] + bz)] = (uint32_t)(si + 1);
+  }
+}
+
+// Gathers node (X, Y, Z)'s contributions in the fixed order: contrib(record, i, j, k, acc)
+// adds the contribution of one particle whose stencil base is the node minus (i, j, k).
+template <int I, int J, int K, int NC, class Contrib>
+__device__ __forceinline__ void det_gather_offset(const ddm_scene &S, const ddm_det &D, int X, int Y, int Z,
+                                                  Contrib &contrib, float *acc) {
+  const int cx = X - I, cy = Y - J, cz = Z - K;
+  if (cx < 0 || cy < 0 || cz < 0) return;
+  const int c = node_index(S, cx, cy, cz);
+  const int n = D.cell_count[c];
+  if (n == 0) return;
+  const int q0 = D.cell_start[c];
+  for (int q = q0; q < q0 + n; ++q) {
+    const float4 *r = reinterpret_cast<const float4 *>(D.records) + 4 * (size_t)D.order[q];
+    contrib.template add<I, J, K>(r[0], r[1], r[2], r[3], acc);
+  }
+}
+template <int NC, class Contrib>
+__device__ __forceinline__ void det_gather_node(const ddm_scene &S, const ddm_det &D, int X, int Y, int Z,
+                                                Contrib &contrib, float *acc) {
+#define DDM_DET_OFF(i, j, k) det_gather_offset<i, j, k, NC>(S, D, X, Y, Z, contrib, acc);
+  DDM_DET_OFF(0, 0, 0) DDM_DET_OFF(0, 0, 1) DDM_DET_OFF(0, 0, 2)
+  DDM_DET_OFF(0, 1, 0) DDM_DET_OFF(0, 1, 1) DDM_DET_OFF(0, 1, 2)
+  DDM_DET_OFF(0, 2, 0) DDM_DET_OFF(0, 2, 1) DDM_DET_OFF(0, 2, 2)
+  DDM_DET_OFF(1, 0, 0) DDM_DET_OFF(1, 0, 1) DDM_DET_OFF(1, 0, 2)
+  DDM_DET_OFF(1, 1, 0) DDM_DET_OFF(1, 1, 1) DDM_DET_OFF(1, 1, 2)
+  DDM_DET_OFF(1, 2, 0) DDM_DET_OFF(1, 2, 1) DDM_DET_OFF(1, 2, 2)
+  DDM_DET_OFF(2, 0, 0) DDM_DET_OFF(2, 0, 1) DDM_DET_OFF(2, 0, 2)
+  DDM_DET_OFF(2, 1, 0) DDM_DET_OFF(2, 1, 1) DDM_DET_OFF(2, 1, 2)
+  DDM_DET_OFF(2, 2, 0) DDM_DET_OFF(2, 2, 1) DDM_DET_OFF(2, 2, 2)
+#undef DDM_DET_OFF
+}
+
 // live chunks / scenes of a launch (a captured launch of fixed size reads them on the device)
 __device__ __forceinline__ int live_chunks(const ddm_batch &B) { return B.live ? B.live[0] : B.n_chunks; }
 __device__ __forceinline__ int live_scenes(const ddm_batch &B) { return B.live ? B.live[1] : B.n_scenes; }
@@ -377,5 +448,11 @@ void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *
                      const ddm_state *next_grad, hipStream_t stream);
 void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
                     hipStream_t stream);
+// deterministic mode: G2P adjoint (per-particle part, then the sorted node gather) and the
+// grid adjoint with per-block body-pose partials reduced in block order
+void launch_g2p_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
+                        const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream);
+void launch_grid_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
+                         const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream);
 
 }  // namespace ddm_fused

@@ -129,9 +129,11 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
 //   tape_status[0] keeps growing, so the taped adjoint sees the overflow).
 // SKIP_RESTORED=true (taped adjoint's recompute): skips the blocks of scenes whose
 //   snapshot was restored.
+// clear_count (deterministic mode, forward): the bin counts of the listed blocks are cleared
+// with the grid (the sorted gather was their last reader).
 template <bool ZERO, bool SAVE, bool SKIP_RESTORED>
 __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                              const ddm_state *__restrict__ next_tab) {
+                                              const ddm_state *__restrict__ next_tab, int32_t *clear_count) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
@@ -165,6 +167,7 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
     if (ZERO) {
       mmv[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
       if (lane == 0) B.block_flags[blk] = 0u;
+      if (clear_count) clear_count[gi] = 0;
     }
   }
 }
@@ -245,6 +248,224 @@ __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__res
   st9(next.C + 9 * (size_t)p, Co);
 }
 
+// ---------------------------------------------------------------- deterministic mode
+// Per particle: k_p2g's particle stage (same modes and outputs), then its stencil record
+// (fx, m | m v, affine) in its launch slot, its base node and its arrival rank among the
+// node's particles; the chunk's blocks are flagged with scene + 1. A restored scene of
+// the taped adjoint is only binned: its grid comes from the snapshot, but the G2P
+// adjoint's sorted gather needs the bins.
+template <int MODE>
+__global__ __launch_bounds__(kWG) void k_det_prep(ddm_batch B, ddm_det D, const ddm_state *__restrict__ cur_tab,
+                                                  const ddm_state *__restrict__ next_tab) {
+  __shared__ int s_box[6];
+  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const ddm_scene S = B.scenes[ch.x];
+  const bool bin_only = MODE == kRecomputeUntaped && S.tape_mmv && S.tape_status[1] == 0;  // uniform
+  const bool temps = MODE == kRecompute || MODE == kRecomputeUntaped || (MODE == kForwardTaped && S.tape_mmv);
+  const ddm_state cur = cur_tab[ch.x];
+  const int p = ch.y + (int)threadIdx.x;
+  const bool valid = p < ch.z;
+  const size_t slot = (size_t)blockIdx.x * kWG + threadIdx.x;
+  const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
+  Stencil st;
+  if (valid) {
+    const f3 x = ld3(cur.x + 3 * (size_t)p);
+    if (bin_only) {
+      stencil(x, S.inv_dx, dim, st);
+    } else {
+      m33 F = ld9(cur.F + 9 * (size_t)p), C = ld9(cur.C + 9 * (size_t)p);
+      m33 Ft = advect_F(F, C, S.dt);
+      m33 U, V;
+      f3 sig;
+      svd3(Ft, U, sig, V);
+      m33 Fn;
+      P2GParticle q;
+      p2g_particle(x, ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+                   ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
+      if (temps) {
+        st9(S.tmp_Ft + 9 * (size_t)p, Ft);
+        st9(S.tmp_U + 9 * (size_t)p, U);
+        st9(S.tmp_V + 9 * (size_t)p, V);
+        st3(S.tmp_sig + 3 * (size_t)p, sig);
+      }
+      if (MODE == kForward || MODE == kForwardTaped) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
+      st = q.st;
+      const m33 &A = q.affine;
+      float4 *r = reinterpret_cast<float4 *>(D.records) + 4 * slot;
+      r[0] = make_float4(st.fx[0], st.fx[1], st.fx[2], q.m);
+      r[1] = make_float4(q.mv.x, q.mv.y, q.mv.z, A.a[0]);
+      r[2] = make_float4(A.a[1], A.a[2], A.a[3], A.a[4]);
+      r[3] = make_float4(A.a[5], A.a[6], A.a[7], A.a[8]);
+    }
+    const int key = node_index(S, st.base[0], st.base[1], st.base[2]);
+    const int rank = atomicAdd(&D.cell_count[key], 1);
+    reinterpret_cast<int2 *>(D.key_slot)[slot] = make_int2(key, rank);
+  } else {
+    st.base[0] = st.base[1] = st.base[2] = 0;
+  }
+  const Tile T = chunk_tile(valid, st.base, s_box);
+  flag_tile_blocks_det(B, S, ch.x, T);
+}
+
+// Block list rebuilt from the flags in block order: flagged blocks per range of kRange
+// blocks, then each range writes its (block, scene) entries at the sum of the counts of
+// the ranges before it. Grid: one workgroup per range.
+constexpr int kRange = 4 * kWG;
+__device__ __forceinline__ int wg_sum_i(int v, int *s_w) {
+  v = wave_sum_i(v);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) t += s_w[w];
+  __syncthreads();
+  return t;
+}
+__global__ __launch_bounds__(kWG) void k_det_count(ddm_batch B, ddm_det D) {
+  __shared__ int s_w[kWG / 64];
+  const int b0 = (int)blockIdx.x * kRange + 4 * (int)threadIdx.x;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c += (b0 + k < B.n_blocks && B.block_flags[b0 + k] != 0u) ? 1 : 0;
+  const int t = wg_sum_i(c, s_w);
+  if (threadIdx.x == 0) D.range_count[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(kWG) void k_det_relist(ddm_batch B, ddm_det D) {
+  __shared__ int s_w[kWG / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int before = 0, total = 0;
+  for (int r = threadIdx.x; r < (int)gridDim.x; r += kWG) {
+    const int c = D.range_count[r];
+    total += c;
+    before += r < (int)blockIdx.x ? c : 0;
+  }
+  before = wg_sum_i(before, s_w);
+  total = wg_sum_i(total, s_w);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = total;
+  const int b0 = (int)blockIdx.x * kRange + 4 * (int)threadIdx.x;
+  uint32_t f[4];
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[k] = b0 + k < B.n_blocks ? B.block_flags[b0 + k] : 0u;
+    c += f[k] != 0u ? 1 : 0;
+  }
+  int incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  int off = before + incl - c;
+#pragma unroll
+  for (int w = 0; w < kWG / 64; ++w) off += w < wave ? s_w[w] : 0;
+  int2 *list = reinterpret_cast<int2 *>(B.block_list);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (f[k] != 0u) list[off++] = make_int2(b0 + k, (int)f[k] - 1);
+}
+
+// Sorted segments: each listed block takes a contiguous range of positions for its 64
+// nodes' particles (ranges in any order; the order within a node is fixed afterwards).
+__global__ __launch_bounds__(kWG) void k_det_scan(ddm_batch B, ddm_det D) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const size_t gi = ((size_t)list[e].x << 6) | (size_t)lane;
+    const int c = D.cell_count[gi];
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    const int tot = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane == 0 && tot > 0) base = atomicAdd(D.counter, tot);
+    base = __shfl(base, 0, 64);
+    D.cell_start[gi] = base + incl - c;
+  }
+}
+__global__ __launch_bounds__(kWG) void k_det_place(ddm_batch B, ddm_det D) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *D.counter = 0;  // the scan has finished
+  if ((int)blockIdx.x >= live_chunks(B)) return;
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int p = ch.y + (int)threadIdx.x;
+  if (p >= ch.z) return;
+  const int slot = (int)blockIdx.x * kWG + (int)threadIdx.x;
+  const int2 ks = reinterpret_cast<const int2 *>(D.key_slot)[slot];
+  const int q = D.cell_start[ks.x] + ks.y;
+  D.order[q] = slot;
+  D.order_p[q] = p;
+}
+// each node's particles into ascending particle index (insertion sort; a node holds few)
+__global__ __launch_bounds__(kWG) void k_det_sort(ddm_batch B, ddm_det D) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const size_t gi = ((size_t)list[e].x << 6) | (size_t)lane;
+    const int n = D.cell_count[gi];
+    if (n < 2) continue;
+    const int q0 = D.cell_start[gi];
+    for (int a = q0 + 1; a < q0 + n; ++a) {
+      const int kp = D.order_p[a], ks = D.order[a];
+      int b = a - 1;
+      while (b >= q0 && D.order_p[b] > kp) {
+        D.order_p[b + 1] = D.order_p[b];
+        D.order[b + 1] = D.order[b];
+        --b;
+      }
+      D.order_p[b + 1] = kp;
+      D.order[b + 1] = ks;
+    }
+  }
+}
+
+// one particle's (m, m v + affine (node - x)) contribution to the node at stencil offset
+// (I, J, K): k_p2g's per-node operations
+struct P2GDetContrib {
+  float dx;
+  template <int I, int J, int K>
+  __device__ __forceinline__ void add(float4 a, float4 b, float4 c, float4 d, float *acc) const {
+    const float wxy = bspline_w<I>(a.x) * bspline_w<J>(a.y);
+    const float d0 = ((float)I - a.x) * dx, d1 = ((float)J - a.y) * dx;
+    const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
+    const float w = wxy * bspline_w<K>(a.z);
+    const float d2 = ((float)K - a.z) * dx;
+    acc[0] += a.w * w;
+    acc[1] += (b.x + (p0 + c.y * d2)) * w;
+    acc[2] += (b.y + (p1 + d.x * d2)) * w;
+    acc[3] += (b.z + (p2 + d.w * d2)) * w;
+  }
+};
+// (m, mv) of every node of the listed blocks, summed in the fixed order (plain stores)
+template <bool SKIP_RESTORED>
+__global__ __launch_bounds__(kWG) void k_det_gather_p2g(ddm_batch B, ddm_det D) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n_listed = *B.block_count;
+  const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+    const int2 entry = list[e];
+    const ddm_scene &S = B.scenes[entry.y];
+    if (SKIP_RESTORED && S.tape_mmv && S.tape_status[1] == 0) continue;
+    const BlockCoord c = block_node(S, entry.x, lane);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    P2GDetContrib k{S.dx};
+    det_gather_node<4>(S, D, c.ix, c.iy, c.iz, k, acc);
+    reinterpret_cast<float4 *>(B.grid_mmv)[((size_t)entry.x << 6) | (size_t)lane] =
+        make_float4(acc[0], acc[1], acc[2], acc[3]);
+  }
+}
+
+__global__ void k_zero_i32(int32_t *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
 __global__ void k_zero(float4 *p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -259,7 +480,28 @@ inline int check(const ddm_batch *b, const void *cur, const void *next) {
   if (!b->scenes || !b->chunks || !b->grid_mmv || !b->grid_v || !b->block_flags || !b->block_list ||
       !b->block_count)
     return DDM_EINVAL;
+  if (b->options & DDM_DETERMINISTIC) {
+    const ddm_det *d = b->det;
+    if (!d || !d->cell_count || !d->cell_start || !d->order || !d->order_p || !d->key_slot || !d->records ||
+        !d->body_partial || !d->range_count || !d->counter)
+      return DDM_EINVAL;
+  }
   return 0;
+}
+
+// deterministic P2G: bin the particles (prep, block list in block order, segments,
+// placement, per-node sort), then gather every listed node in the fixed order
+template <int MODE>
+void launch_det_p2g(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream) {
+  const ddm_det &d = *b->det;
+  const int ranges = b->n_blocks > 0 ? (b->n_blocks + kRange - 1) / kRange : 1;
+  k_det_prep<MODE><<<b->n_chunks, kWG, 0, stream>>>(*b, d, cur, next);
+  k_det_count<<<ranges, kWG, 0, stream>>>(*b, d);
+  k_det_relist<<<ranges, kWG, 0, stream>>>(*b, d);
+  k_det_scan<<<grid_wgs(b), kWG, 0, stream>>>(*b, d);
+  k_det_place<<<b->n_chunks, kWG, 0, stream>>>(*b, d);
+  k_det_sort<<<grid_wgs(b), kWG, 0, stream>>>(*b, d);
+  k_det_gather_p2g<MODE == kRecomputeUntaped><<<grid_wgs(b), kWG, 0, stream>>>(*b, d);
 }
 
 }  // namespace
@@ -278,14 +520,34 @@ static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_st
   int e = check(b, cur, next);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
+  const bool det = b->options & DDM_DETERMINISTIC;
   mark(ev, 0, stream);
-  k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  if (det) launch_det_p2g<kForward>(b, cur, next, stream);
+  else k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<true, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
+  k_grid<true, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
   return (int)hipGetLastError();
+}
+
+// adjoint kernels after the grid update (both adjoint launch sequences)
+static void launch_adjoint_tail(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
+                                hipEvent_t *ev, int ev0) {
+  if (b->options & DDM_DETERMINISTIC) {
+    launch_g2p_adj_det(*b, *b->det, cur, next, cur_grad, next_grad, stream);
+    mark(ev, ev0, stream);
+    launch_grid_adj_det(*b, *b->det, cur, next, cur_grad, next_grad, stream);
+  } else {
+    launch_g2p_adj(*b, cur, next, cur_grad, next_grad, stream);
+    mark(ev, ev0, stream);
+    launch_grid_adj(*b, cur, next, cur_grad, next_grad, stream);
+  }
+  mark(ev, ev0 + 1, stream);
+  launch_p2g_adj(*b, cur, cur_grad, next_grad, stream);
+  mark(ev, ev0 + 2, stream);
 }
 
 static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
@@ -295,16 +557,12 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
   if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  if (b->options & DDM_DETERMINISTIC) launch_det_p2g<kRecompute>(b, cur, next, stream);
+  else k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<false, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
+  k_grid<false, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   mark(ev, 2, stream);
-  launch_g2p_adj(*b, cur, next, cur_grad, next_grad, stream);
-  mark(ev, 3, stream);
-  launch_grid_adj(*b, cur, next, cur_grad, next_grad, stream);
-  mark(ev, 4, stream);
-  launch_p2g_adj(*b, cur, cur_grad, next_grad, stream);
-  mark(ev, 5, stream);
+  launch_adjoint_tail(b, cur, next, cur_grad, next_grad, stream, ev, 3);
   return (int)hipGetLastError();
 }
 
@@ -313,11 +571,13 @@ static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const 
   int e = check(b, cur, next);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
+  const bool det = b->options & DDM_DETERMINISTIC;
   mark(ev, 0, stream);
   k_tape_reset<<<1, kWG, 0, stream>>>(*b);
-  k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  if (det) launch_det_p2g<kForwardTaped>(b, cur, next, stream);
+  else k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
-  k_grid<true, true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
+  k_grid<true, true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
@@ -333,15 +593,11 @@ static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
   k_restore<<<dim3(kRestoreWGs, b->n_scenes), kWG, 0, stream>>>(*b);
-  k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
-  k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next);
+  if (b->options & DDM_DETERMINISTIC) launch_det_p2g<kRecomputeUntaped>(b, cur, next, stream);
+  else k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   mark(ev, 1, stream);
-  launch_g2p_adj(*b, cur, next, cur_grad, next_grad, stream);
-  mark(ev, 2, stream);
-  launch_grid_adj(*b, cur, next, cur_grad, next_grad, stream);
-  mark(ev, 3, stream);
-  launch_p2g_adj(*b, cur, cur_grad, next_grad, stream);
-  mark(ev, 4, stream);
+  launch_adjoint_tail(b, cur, next, cur_grad, next_grad, stream, ev, 2);
   return (int)hipGetLastError();
 }
 
@@ -401,5 +657,7 @@ DDM_API int ddm_batch_reset(const ddm_batch *b, hipStream_t stream) {
   if (b->grid_dmmv) k_zero<<<wg, kWG, 0, stream>>>(reinterpret_cast<float4 *>(b->grid_dmmv), nodes);
   k_zero_u32<<<wg, kWG, 0, stream>>>(b->block_flags, (size_t)b->n_blocks);
   if (b->block_count) k_zero_u32<<<1, kWG, 0, stream>>>(reinterpret_cast<uint32_t *>(b->block_count), 1);
+  if (b->det && b->det->cell_count) k_zero_i32<<<wg, kWG, 0, stream>>>(b->det->cell_count, nodes);
+  if (b->det && b->det->counter) k_zero_i32<<<1, kWG, 0, stream>>>(b->det->counter, 1);
   return (int)hipGetLastError();
 }

@@ -122,7 +122,12 @@ def state_table(records, device) -> torch.Tensor:
 
 
 class Engine:
-    def __init__(self, scenes: List[SceneSpec], device="cuda:0", with_grad: bool = True):
+    def __init__(self, scenes: List[SceneSpec], device="cuda:0", with_grad: bool = True,
+                 deterministic: bool = False):
+        """deterministic=True: every substep (forward and adjoint) uses sorted reductions
+        instead of atomics, so results are bitwise reproducible and identical for a scene
+        alone and in any batch (slower; needs a workspace of about 90 B per particle
+        and 512 B per node block plus 64 B per block and body)."""
         if not scenes:
             raise ValueError("an engine needs at least one scene")
         for s in scenes:
@@ -172,6 +177,20 @@ class Engine:
             self.tmp_sig = z(P, 3)
         else:
             self.grid_dv = self.grid_dmmv = self.tmp_Ft = self.tmp_U = self.tmp_V = self.tmp_sig = None
+
+        self.deterministic = bool(deterministic)
+        self._det = None
+        if self.deterministic:
+            slots = max(sum((c + CHUNK - 1) // CHUNK for c in caps), 1) * CHUNK
+            i32 = lambda n: torch.zeros(int(n), dtype=torch.int32, device=dev)
+            d = dict(cell_count=i32(G), cell_start=i32(G), order=i32(slots), order_p=i32(slots),
+                     key_slot=i32(2 * slots), records=z(slots, 16),
+                     body_partial=z(self.n_blocks * max(max(nbs), 1), 16),
+                     range_count=i32((self.n_blocks + 1023) // 1024), counter=i32(1))
+            self.det_buffers = d
+            self._det = _lib.DdmDet(*[_lib.ptr(d[k]) for k in ("cell_count", "cell_start", "order", "order_p",
+                                                               "key_slot", "records", "body_partial",
+                                                               "range_count", "counter")])
 
         tab = np.zeros(len(scenes), dtype=_lib.SCENE_DTYPE)
         f4 = 4
@@ -231,7 +250,11 @@ class Engine:
         self._batch = _lib.DdmBatch(
             len(self.scenes), self.n_chunks, self.n_blocks, max(s.n_bodies for s in self.scenes),
             p(self.scene_table), p(self.chunks), p(self.grid_mmv), p(self.grid_v), p(self.grid_dv), p(self.grid_dmmv),
-            p(self.block_flags), p(self.block_list), p(self.block_count))
+            p(self.block_flags), p(self.block_list), p(self.block_count), None,
+            ctypes.pointer(self._det) if self._det is not None else None, self._options())
+
+    def _options(self, extra: int = 0) -> int:
+        return (_lib.DDM_DETERMINISTIC if self.deterministic else 0) | extra
 
     def table(self, x, v, F, C, body_pos, body_rot) -> torch.Tensor:
         """Per-scene ddm_state records for concatenated tensors (cached by address)."""
@@ -334,7 +357,7 @@ class Engine:
         b = _lib.DdmBatch.from_buffer_copy(self._batch)
         if scene_tab is not None:
             b.scenes = scene_tab.data_ptr()
-        b.options = options
+        b.options = self._options(options)
         return b
 
     def _batch_with_scenes(self, scene_tab):
@@ -361,5 +384,8 @@ class Engine:
 
     def nbytes(self) -> int:
         ts = [self.mass, self.vol, self.mu_lam_yield, self.body_prim, self.body_args, self.grid_mmv, self.grid_v,
-              self.block_flags, self.block_list, self.block_count, self.grid_dv, self.grid_dmmv, self.tmp_Ft, self.tmp_U, self.tmp_V, self.tmp_sig]
+              self.block_flags, self.block_list, self.block_count, self.grid_dv, self.grid_dmmv, self.tmp_Ft, self.tmp_U,
+              self.tmp_V, self.tmp_sig]
+        if self._det is not None:
+            ts += list(self.det_buffers.values())
         return int(sum(t.numel() * t.element_size() for t in ts if t is not None))

@@ -510,7 +510,7 @@ class RolloutServer:
                               A[kind, "chunks"], self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
                               self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
                               self.block_list.data_ptr(), self.block_count.data_ptr(),
-                              A[kind, "live"] if live else None, _lib.DDM_GRAD_OVERWRITE)
+                              live=A[kind, "live"] if live else None, options=_lib.DDM_GRAD_OVERWRITE)
         st = _lib.stream_ptr()
         if kind == "F":  # scenes without a tape in their record run a plain forward substep
             _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(batch), A["F", "cur"], A["F", "nxt"], st),
@@ -557,8 +557,10 @@ class RolloutServer:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(self.capture_stream):
                 g.capture_begin(capture_error_mode="thread_local")
-                self._launch_tick(key, live=True)
-                g.capture_end()
+                try:
+                    self._launch_tick(key, live=True)
+                finally:  # a failed launch must not leave the stream capturing
+                    g.capture_end()
             self.graphs[key] = g
         g.replay()
         self.replays += 1

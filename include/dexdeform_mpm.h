@@ -108,6 +108,7 @@ int particle2mass(float *particle_x, float *particle_m, int *grid_lower, const i
 
 #define DDM_MAX_BODIES 64
 #define DDM_GRAD_OVERWRITE 1  /* ddm_batch.options */
+#define DDM_DETERMINISTIC 2   /* ddm_batch.options; needs ddm_batch.det */
 
 typedef struct ddm_scene {
   int32_t p_count;             /* particles of the scene (chunks define the work) */
@@ -135,6 +136,29 @@ typedef struct ddm_scene {
   int32_t reserved;
 } ddm_scene;                   /* 176 bytes */
 
+/* Workspace of the deterministic sorted-reduction mode (ddm_batch.options bit
+ * DDM_DETERMINISTIC; no counterpart in the reference, whose atomics make every run's
+ * rounding differ). Each substep bins the launch's particles by stencil-base node
+ * (counting sort, particles of a node in ascending particle index), and every grid node
+ * sums its particles' contributions in that fixed order (27 stencil offsets, then the
+ * node's particles) instead of through atomics; body-pose gradients are reduced per node
+ * block and then over blocks in block order. Results are then bitwise reproducible, and
+ * identical for a scene alone and in any batch. "Launch slot" = chunk * 256 + position
+ * in the chunk. */
+typedef struct ddm_det {
+  int32_t *cell_count;         /* [64*n_blocks] particles per stencil-base node; zero between
+                                  substeps (ddm_batch_reset clears it) */
+  int32_t *cell_start;         /* [64*n_blocks] first sorted position of the node's particles */
+  int32_t *order;              /* [256*n_chunks] launch slots sorted by base node, then particle index */
+  int32_t *order_p;            /* [256*n_chunks] particle index of each sorted entry */
+  int32_t *key_slot;           /* [256*n_chunks][2] base node and arrival rank per launch slot */
+  float *records;              /* [256*n_chunks][16] per-particle stencil record per launch slot */
+  float *body_partial;         /* [n_blocks][max_bodies][16] per-block body-pose gradients */
+  int32_t *range_count;        /* [(n_blocks + 1023) / 1024] flagged blocks per range */
+  int32_t *counter;            /* [1] sorted-segment allocator; zero between substeps */
+  int32_t reserved[2];
+} ddm_det;                     /* 80 bytes */
+
 typedef struct ddm_batch {
   int32_t n_scenes;
   int32_t n_chunks;            /* particle chunks of <= 256 particles, none straddling scenes */
@@ -155,8 +179,11 @@ typedef struct ddm_batch {
   const int32_t *live;         /* optional device [2] = (chunks, scenes) actually used, <= n_chunks
                                   and n_scenes: lets one captured launch of fixed size serve
                                   batches of any size (NULL = n_chunks, n_scenes) */
+  const ddm_det *det;          /* workspace (host pointer to a record whose arrays are device
+                                  memory) for DDM_DETERMINISTIC; NULL otherwise */
   int32_t options;             /* DDM_GRAD_OVERWRITE: the adjoint substep writes the current
-                                  state's x/v/F/C gradients instead of adding to them */
+                                  state's x/v/F/C gradients instead of adding to them;
+                                  DDM_DETERMINISTIC: sorted reductions instead of atomics */
   int32_t reserved;
 } ddm_batch;
 

@@ -38,6 +38,7 @@ def test_struct_layouts_match_header():
     from dexdeform_amd import _lib
     assert ctypes.sizeof(_lib.DdmScene) == 176
     assert ctypes.sizeof(_lib.DdmState) == 6 * 8
-    assert ctypes.sizeof(_lib.DdmBatch) == 16 + 10 * 8 + 8
+    assert ctypes.sizeof(_lib.DdmBatch) == 16 + 11 * 8 + 8
+    assert ctypes.sizeof(_lib.DdmDet) == 9 * 8 + 8
     hdr = open(HEADER).read()
     assert "} ddm_scene;                   /* 176 bytes */" in hdr

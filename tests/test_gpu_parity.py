@@ -75,7 +75,7 @@ def test_fused_forward_one_substep(plain):
     for name, g, r in zip("xvFC", got, (ref.x, ref.v, ref.F, ref.C)):
         e = rel_err(g, r)
         print(f"substep {name}: rel err {e:.3e}")
-        assert e <= 1e-4, (name, e)
+        assert e <= 1e-5, (name, e)
 
 
 def test_fused_trajectory_50(plain):
