@@ -80,11 +80,11 @@ def make_batched(n_scenes, seed, T):
     return scenes, poses
 
 
-def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False):
+def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False, deterministic=False):
     import torch
     from dexdeform_amd.mpm.engine import Engine
     from dexdeform_amd.mpm.rollout import Rollout
-    eng = Engine([s.spec for s in scenes], device=device, with_grad=True)
+    eng = Engine([s.spec for s in scenes], device=device, with_grad=True, deterministic=deterministic)
     ro = Rollout(eng, T, checkpoint_every=checkpoint_every, tape=tape)
     x0 = torch.tensor(np.concatenate([s.x0 for s in scenes]), device=device)
     ro.set_initial(x0)
@@ -255,6 +255,8 @@ def main():
                          "reading the forward pass's tape")
     ap.add_argument("--no-recompute", action="store_true",
                     help="skip the extra measurement of the recompute adjoint")
+    ap.add_argument("--no-det", action="store_true",
+                    help="skip the extra measurement of the deterministic sorted-reduction mode")
     ap.add_argument("--serve-rates", default="10,50,100,200", help="request arrival rates (req/s) to sweep")
     ap.add_argument("--serve-eager", action="store_true", help="serve without capturing ticks in HIP graphs")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
@@ -325,6 +327,21 @@ def main():
         log(f"batched graph, recompute adjoint: {recompute_value:.3e} particle-substeps/s")
         del rr, gr_
         torch.cuda.empty_cache()
+    det_value = None
+    if not args.no_det:
+        # the same rollout in the deterministic sorted-reduction mode (bitwise reproducible)
+        _, rd, gd_ = build_rollout(scenes, poses, T, dev, tape=tape, deterministic=True)
+        rd.capture(gd_)
+        rd.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            rd.replay()
+            torch.cuda.synchronize()
+        det_value = n_total * T / ((time.perf_counter() - t0) / args.steps)
+        log(f"batched graph, deterministic mode: {det_value:.3e} particle-substeps/s")
+        del rd, gd_
+        torch.cuda.empty_cache()
     kt = KernelTimer(ro).timed_step(gx)
     log("per-kernel ms/launch: " + ", ".join(f"{k}={v:.4f}" for k, v in kt.items()))
     kb = KERNEL_BYTES_TAPED if tape else KERNEL_BYTES
@@ -389,6 +406,7 @@ def main():
                    "global_batch": len(scenes), "substeps": T, "particles": n_total, "parallelism": "none"},
         "adjoint": "taped" if tape else "recompute",
         "recompute_value": recompute_value,
+        "deterministic_value": det_value,
         "eager_value": eager_value,
         "plain_value": plain_value,
         "plain_eager_value": plain_eager_value,

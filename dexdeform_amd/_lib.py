@@ -57,8 +57,8 @@ class DdmDet(ctypes.Structure):
     """Workspace of the deterministic sorted-reduction mode (device arrays)."""
     _fields_ = [
         ("cell_count", c_void_p), ("cell_start", c_void_p), ("order", c_void_p), ("order_p", c_void_p),
-        ("key_slot", c_void_p), ("records", c_void_p), ("body_partial", c_void_p), ("range_count", c_void_p),
-        ("counter", c_void_p), ("reserved", c_int32 * 2),
+        ("key_slot", c_void_p), ("records", c_void_p), ("sorted", c_void_p), ("body_partial", c_void_p),
+        ("range_count", c_void_p), ("counter", c_void_p),
     ]
 
 

@@ -275,8 +275,8 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
 }
 
 // Deterministic G2P adjoint, per particle: the x adjoint as k_g2p_adj, and the record
-// (fx | node-velocity adjoint seed, C adjoint) in the particle's launch slot for the
-// sorted gather (the bins are the P2G pass's: same positions, same stencil bases).
+// (fx | node-velocity adjoint seed, C adjoint) for the sorted gather (the bins are the P2G
+// pass's: same positions, same stencil bases).
 __global__ __launch_bounds__(kWG) void k_g2p_adj_det(ddm_batch B, ddm_det D, const ddm_state *__restrict__ cur_tab,
                                                      const ddm_state *__restrict__ next_tab,
                                                      const ddm_state *__restrict__ cur_grad_tab,
@@ -299,7 +299,9 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj_det(ddm_batch B, ddm_det D, con
                             dim, &g_nv, &st);
   if (B.options & DDM_GRAD_OVERWRITE) st3(cur_grad.x + 3 * (size_t)p, gxp);
   else add3(cur_grad.x + 3 * (size_t)p, gxp);
-  float4 *r = reinterpret_cast<float4 *>(D.records) + 4 * ((size_t)blockIdx.x * kWG + threadIdx.x);
+  // at the particle's sorted position (the P2G pass of this adjoint sorted the same bins)
+  const size_t slot = (size_t)blockIdx.x * kWG + threadIdx.x;
+  float4 *r = reinterpret_cast<float4 *>(D.sorted) + 4 * (size_t)D.key_slot[2 * slot + 1];
   r[0] = make_float4(st.fx[0], st.fx[1], st.fx[2], 0.f);
   r[1] = make_float4(g_nv.x, g_nv.y, g_nv.z, gC.a[0]);
   r[2] = make_float4(gC.a[1], gC.a[2], gC.a[3], gC.a[4]);
@@ -321,19 +323,20 @@ struct G2PAdjDetContrib {
     acc[2] += w * b.z + (p2 + d.w * d2) * xx;
   }
 };
+// the node-velocity adjoint of every node of the listed blocks; one workgroup per block
 __global__ __launch_bounds__(kWG) void k_det_gather_g2p_adj(ddm_batch B, ddm_det D) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ DetGatherLDS L;
   const int n_listed = *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
-  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+  for (int e = blockIdx.x; e < n_listed; e += gridDim.x) {  // uniform over the workgroup
     const int2 entry = list[e];
     const ddm_scene &S = B.scenes[entry.y];
-    const BlockCoord c = block_node(S, entry.x, lane);
-    float acc[3] = {0.f, 0.f, 0.f};
     G2PAdjDetContrib k{S.inv_dx};
-    det_gather_node<3>(S, D, c.ix, c.iy, c.iz, k, acc);
-    reinterpret_cast<float4 *>(B.grid_dv)[((size_t)entry.x << 6) | (size_t)lane] =
-        make_float4(acc[0], acc[1], acc[2], 0.f);
+    float acc[3];
+    det_gather_block<3>(S, D, entry.x, L, k, acc);
+    if (threadIdx.x < 64)
+      reinterpret_cast<float4 *>(B.grid_dv)[((size_t)entry.x << 6) | threadIdx.x] =
+          make_float4(acc[0], acc[1], acc[2], 0.f);
   }
 }
 
@@ -384,7 +387,7 @@ void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *
 void launch_g2p_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
                         const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
   k_g2p_adj_det<<<b.n_chunks, kWG, 0, stream>>>(b, d, cur, next, cur_grad, next_grad);
-  k_det_gather_g2p_adj<<<grid_wgs(&b), kWG, 0, stream>>>(b, d);
+  k_det_gather_g2p_adj<<<grid_wgs(&b) * 4, kWG, 0, stream>>>(b, d);
 }
 void launch_grid_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
                          const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {

@@ -401,36 +401,76 @@ __device__ void flag_tile_blocks_det(const ddm_batch &B, const ddm_scene &S, int
   }
 }
 
-// Gathers node (X, Y, Z)'s contributions in the fixed order: contrib(record, i, j, k, acc)
-// adds the contribution of one particle whose stencil base is the node minus (i, j, k).
-template <int I, int J, int K, int NC, class Contrib>
-__device__ __forceinline__ void det_gather_offset(const ddm_scene &S, const ddm_det &D, int X, int Y, int Z,
+// Sorted gather of one listed block by one workgroup. The (count, start) of the 6x6x6
+// stencil-base nodes that can reach the block are staged in LDS; wave w of the workgroup
+// sums, for every node of the block (lane = node), the stencil offsets with index
+// = w (mod 4) in (i, j, k) order, each over the offset's base node's particles in sorted
+// order (their records are contiguous); the four partial sums are then added in wave
+// order. contrib.add<I, J, K>(record, acc) adds one particle whose stencil base is the
+// node minus (I, J, K).
+struct DetGatherLDS {
+  int cnt[216], start[216];
+  float part[kWG / 64][64][4];
+};
+template <int I, int J, int K, class Contrib>
+__device__ __forceinline__ void det_gather_offset(const DetGatherLDS &L, const ddm_det &D, int lx, int ly, int lz,
                                                   Contrib &contrib, float *acc) {
-  const int cx = X - I, cy = Y - J, cz = Z - K;
-  if (cx < 0 || cy < 0 || cz < 0) return;
-  const int c = node_index(S, cx, cy, cz);
-  const int n = D.cell_count[c];
+  const int c = ((lx - I + 2) * 6 + (ly - J + 2)) * 6 + (lz - K + 2);
+  const int n = L.cnt[c];
   if (n == 0) return;
-  const int q0 = D.cell_start[c];
-  for (int q = q0; q < q0 + n; ++q) {
-    const float4 *r = reinterpret_cast<const float4 *>(D.records) + 4 * (size_t)D.order[q];
-    contrib.template add<I, J, K>(r[0], r[1], r[2], r[3], acc);
+  const float4 *r = reinterpret_cast<const float4 *>(D.sorted) + 4 * (size_t)L.start[c];
+  const float4 *const re = r + 4 * (size_t)n;
+  for (; r + 4 < re; r += 8) {  // two records per iteration (loads overlap), summed in order
+    const float4 a0 = r[0], b0 = r[1], c0 = r[2], d0 = r[3], a1 = r[4], b1 = r[5], c1 = r[6], d1 = r[7];
+    contrib.template add<I, J, K>(a0, b0, c0, d0, acc);
+    contrib.template add<I, J, K>(a1, b1, c1, d1, acc);
   }
+  if (r < re) contrib.template add<I, J, K>(r[0], r[1], r[2], r[3], acc);
 }
+// returns, in the lanes of wave 0, the block's node sums (NC components); call with the
+// whole workgroup (it synchronizes)
 template <int NC, class Contrib>
-__device__ __forceinline__ void det_gather_node(const ddm_scene &S, const ddm_det &D, int X, int Y, int Z,
-                                                Contrib &contrib, float *acc) {
-#define DDM_DET_OFF(i, j, k) det_gather_offset<i, j, k, NC>(S, D, X, Y, Z, contrib, acc);
-  DDM_DET_OFF(0, 0, 0) DDM_DET_OFF(0, 0, 1) DDM_DET_OFF(0, 0, 2)
-  DDM_DET_OFF(0, 1, 0) DDM_DET_OFF(0, 1, 1) DDM_DET_OFF(0, 1, 2)
-  DDM_DET_OFF(0, 2, 0) DDM_DET_OFF(0, 2, 1) DDM_DET_OFF(0, 2, 2)
-  DDM_DET_OFF(1, 0, 0) DDM_DET_OFF(1, 0, 1) DDM_DET_OFF(1, 0, 2)
-  DDM_DET_OFF(1, 1, 0) DDM_DET_OFF(1, 1, 1) DDM_DET_OFF(1, 1, 2)
-  DDM_DET_OFF(1, 2, 0) DDM_DET_OFF(1, 2, 1) DDM_DET_OFF(1, 2, 2)
-  DDM_DET_OFF(2, 0, 0) DDM_DET_OFF(2, 0, 1) DDM_DET_OFF(2, 0, 2)
-  DDM_DET_OFF(2, 1, 0) DDM_DET_OFF(2, 1, 1) DDM_DET_OFF(2, 1, 2)
-  DDM_DET_OFF(2, 2, 0) DDM_DET_OFF(2, 2, 1) DDM_DET_OFF(2, 2, 2)
+__device__ void det_gather_block(const ddm_scene &S, const ddm_det &D, int blk, DetGatherLDS &L, Contrib &contrib,
+                                 float *out) {
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int lb = blk - S.blk_begin, nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
+  const int bx = 4 * (lb / (nby * nbz)), by = 4 * ((lb / nbz) % nby), bz = 4 * (lb % nbz);
+  if (t < 216) {
+    const int x = bx - 2 + t / 36, y = by - 2 + (t / 6) % 6, z = bz - 2 + t % 6;
+    int n = 0, q = 0;
+    if (x >= 0 && y >= 0 && z >= 0 && x < S.dim[0] && y < S.dim[1] && z < S.dim[2]) {
+      const int c = node_index(S, x, y, z);
+      n = D.cell_count[c];
+      if (n) q = D.cell_start[c];
+    }
+    L.cnt[t] = n;
+    L.start[t] = q;
+  }
+  __syncthreads();
+  const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+  float acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.f;
+#define DDM_DET_OFF(n, i, j, k) \
+  if (wave == (n) % 4) det_gather_offset<i, j, k>(L, D, lx, ly, lz, contrib, acc);
+  DDM_DET_OFF(0, 0, 0, 0) DDM_DET_OFF(1, 0, 0, 1) DDM_DET_OFF(2, 0, 0, 2)
+  DDM_DET_OFF(3, 0, 1, 0) DDM_DET_OFF(4, 0, 1, 1) DDM_DET_OFF(5, 0, 1, 2)
+  DDM_DET_OFF(6, 0, 2, 0) DDM_DET_OFF(7, 0, 2, 1) DDM_DET_OFF(8, 0, 2, 2)
+  DDM_DET_OFF(9, 1, 0, 0) DDM_DET_OFF(10, 1, 0, 1) DDM_DET_OFF(11, 1, 0, 2)
+  DDM_DET_OFF(12, 1, 1, 0) DDM_DET_OFF(13, 1, 1, 1) DDM_DET_OFF(14, 1, 1, 2)
+  DDM_DET_OFF(15, 1, 2, 0) DDM_DET_OFF(16, 1, 2, 1) DDM_DET_OFF(17, 1, 2, 2)
+  DDM_DET_OFF(18, 2, 0, 0) DDM_DET_OFF(19, 2, 0, 1) DDM_DET_OFF(20, 2, 0, 2)
+  DDM_DET_OFF(21, 2, 1, 0) DDM_DET_OFF(22, 2, 1, 1) DDM_DET_OFF(23, 2, 1, 2)
+  DDM_DET_OFF(24, 2, 2, 0) DDM_DET_OFF(25, 2, 2, 1) DDM_DET_OFF(26, 2, 2, 2)
 #undef DDM_DET_OFF
+#pragma unroll
+  for (int c = 0; c < NC; ++c) L.part[wave][lane][c] = acc[c];
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = ((L.part[0][lane][c] + L.part[1][lane][c]) + L.part[2][lane][c]) + L.part[3][lane][c];
+  }
+  __syncthreads();  // LDS is reused by the next block
 }
 
 // live chunks / scenes of a launch (a captured launch of fixed size reads them on the device)

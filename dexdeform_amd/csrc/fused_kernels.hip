@@ -402,27 +402,55 @@ __global__ __launch_bounds__(kWG) void k_det_place(ddm_batch B, ddm_det D) {
   D.order[q] = slot;
   D.order_p[q] = p;
 }
-// each node's particles into ascending particle index (insertion sort; a node holds few)
-__global__ __launch_bounds__(kWG) void k_det_sort(ddm_batch B, ddm_det D) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// Final order: per listed block (one workgroup), every entry's rank among its node's
+// entries by particle index (particle indices are unique within a scene, and a node holds
+// one scene's particles), counted over the node's entries held in LDS (read from global
+// memory when the block's segment exceeds kSortCap). Each particle's record is copied to
+// its sorted position, which replaces the arrival rank in key_slot.
+constexpr int kSortCap = 3072;
+__global__ __launch_bounds__(kWG) void k_det_order(ddm_batch B, ddm_det D) {
+  __shared__ int s_slot[kSortCap], s_p[kSortCap];
+  __shared__ int s_start[64], s_cnt[64];
+  const int t = threadIdx.x;
   const int n_listed = *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
-  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
-    const size_t gi = ((size_t)list[e].x << 6) | (size_t)lane;
-    const int n = D.cell_count[gi];
-    if (n < 2) continue;
-    const int q0 = D.cell_start[gi];
-    for (int a = q0 + 1; a < q0 + n; ++a) {
-      const int kp = D.order_p[a], ks = D.order[a];
-      int b = a - 1;
-      while (b >= q0 && D.order_p[b] > kp) {
-        D.order_p[b + 1] = D.order_p[b];
-        D.order[b + 1] = D.order[b];
-        --b;
-      }
-      D.order_p[b + 1] = kp;
-      D.order[b + 1] = ks;
+  for (int e = blockIdx.x; e < n_listed; e += gridDim.x) {  // uniform over the workgroup
+    const size_t g0 = (size_t)list[e].x << 6;
+    if (t < 64) {
+      s_start[t] = D.cell_start[g0 + t];
+      s_cnt[t] = D.cell_count[g0 + t];
     }
+    __syncthreads();
+    const int base = s_start[0], tot = s_start[63] + s_cnt[63] - s_start[0];
+    const bool fits = tot <= kSortCap;
+    if (fits)
+      for (int i = t; i < tot; i += kWG) {
+        s_slot[i] = D.order[base + i];
+        s_p[i] = D.order_p[base + i];
+      }
+    __syncthreads();
+    const int *P = fits ? s_p : D.order_p + base;
+    const int *L = fits ? s_slot : D.order + base;
+    for (int i = t; i < tot; i += kWG) {
+      int lo = 0, hi = 63;  // node of entry i: the last node whose segment starts at or before i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] - base <= i) lo = mid;
+        else hi = mid - 1;
+      }
+      const int c0 = s_start[lo] - base, n = s_cnt[lo], pi = P[i];
+      int r = 0;
+      for (int j = c0; j < c0 + n; ++j) r += P[j] < pi ? 1 : 0;
+      const int q = base + c0 + r, slot = L[i];
+      const float4 *src = reinterpret_cast<const float4 *>(D.records) + 4 * (size_t)slot;
+      float4 *dst = reinterpret_cast<float4 *>(D.sorted) + 4 * (size_t)q;
+      dst[0] = src[0];
+      dst[1] = src[1];
+      dst[2] = src[2];
+      dst[3] = src[3];
+      D.key_slot[2 * (size_t)slot + 1] = q;
+    }
+    __syncthreads();  // LDS is reused by the next block
   }
 }
 
@@ -443,22 +471,23 @@ struct P2GDetContrib {
     acc[3] += (b.z + (p2 + d.w * d2)) * w;
   }
 };
-// (m, mv) of every node of the listed blocks, summed in the fixed order (plain stores)
+// (m, mv) of every node of the listed blocks, summed in the fixed order (plain stores);
+// one workgroup per listed block
 template <bool SKIP_RESTORED>
 __global__ __launch_bounds__(kWG) void k_det_gather_p2g(ddm_batch B, ddm_det D) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ DetGatherLDS L;
   const int n_listed = *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
-  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
+  for (int e = blockIdx.x; e < n_listed; e += gridDim.x) {  // uniform over the workgroup
     const int2 entry = list[e];
     const ddm_scene &S = B.scenes[entry.y];
     if (SKIP_RESTORED && S.tape_mmv && S.tape_status[1] == 0) continue;
-    const BlockCoord c = block_node(S, entry.x, lane);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
     P2GDetContrib k{S.dx};
-    det_gather_node<4>(S, D, c.ix, c.iy, c.iz, k, acc);
-    reinterpret_cast<float4 *>(B.grid_mmv)[((size_t)entry.x << 6) | (size_t)lane] =
-        make_float4(acc[0], acc[1], acc[2], acc[3]);
+    float acc[4];
+    det_gather_block<4>(S, D, entry.x, L, k, acc);
+    if (threadIdx.x < 64)
+      reinterpret_cast<float4 *>(B.grid_mmv)[((size_t)entry.x << 6) | threadIdx.x] =
+          make_float4(acc[0], acc[1], acc[2], acc[3]);
   }
 }
 
@@ -500,8 +529,8 @@ void launch_det_p2g(const ddm_batch *b, const ddm_state *cur, const ddm_state *n
   k_det_relist<<<ranges, kWG, 0, stream>>>(*b, d);
   k_det_scan<<<grid_wgs(b), kWG, 0, stream>>>(*b, d);
   k_det_place<<<b->n_chunks, kWG, 0, stream>>>(*b, d);
-  k_det_sort<<<grid_wgs(b), kWG, 0, stream>>>(*b, d);
-  k_det_gather_p2g<MODE == kRecomputeUntaped><<<grid_wgs(b), kWG, 0, stream>>>(*b, d);
+  k_det_order<<<grid_wgs(b) * 4, kWG, 0, stream>>>(*b, d);
+  k_det_gather_p2g<MODE == kRecomputeUntaped><<<grid_wgs(b) * 4, kWG, 0, stream>>>(*b, d);
 }
 
 }  // namespace

@@ -126,7 +126,7 @@ class Engine:
                  deterministic: bool = False):
         """deterministic=True: every substep (forward and adjoint) uses sorted reductions
         instead of atomics, so results are bitwise reproducible and identical for a scene
-        alone and in any batch (slower; needs a workspace of about 90 B per particle
+        alone and in any batch (slower; needs a workspace of about 150 B per particle
         and 512 B per node block plus 64 B per block and body)."""
         if not scenes:
             raise ValueError("an engine needs at least one scene")
@@ -184,12 +184,12 @@ class Engine:
             slots = max(sum((c + CHUNK - 1) // CHUNK for c in caps), 1) * CHUNK
             i32 = lambda n: torch.zeros(int(n), dtype=torch.int32, device=dev)
             d = dict(cell_count=i32(G), cell_start=i32(G), order=i32(slots), order_p=i32(slots),
-                     key_slot=i32(2 * slots), records=z(slots, 16),
+                     key_slot=i32(2 * slots), records=z(slots, 16), sorted=z(slots, 16),
                      body_partial=z(self.n_blocks * max(max(nbs), 1), 16),
                      range_count=i32((self.n_blocks + 1023) // 1024), counter=i32(1))
             self.det_buffers = d
             self._det = _lib.DdmDet(*[_lib.ptr(d[k]) for k in ("cell_count", "cell_start", "order", "order_p",
-                                                               "key_slot", "records", "body_partial",
+                                                               "key_slot", "records", "sorted", "body_partial",
                                                                "range_count", "counter")])
 
         tab = np.zeros(len(scenes), dtype=_lib.SCENE_DTYPE)

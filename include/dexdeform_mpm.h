@@ -149,14 +149,15 @@ typedef struct ddm_det {
   int32_t *cell_count;         /* [64*n_blocks] particles per stencil-base node; zero between
                                   substeps (ddm_batch_reset clears it) */
   int32_t *cell_start;         /* [64*n_blocks] first sorted position of the node's particles */
-  int32_t *order;              /* [256*n_chunks] launch slots sorted by base node, then particle index */
-  int32_t *order_p;            /* [256*n_chunks] particle index of each sorted entry */
-  int32_t *key_slot;           /* [256*n_chunks][2] base node and arrival rank per launch slot */
+  int32_t *order;              /* [256*n_chunks] launch slots in arrival order within each node */
+  int32_t *order_p;            /* [256*n_chunks] particle index of each of those entries */
+  int32_t *key_slot;           /* [256*n_chunks][2] per launch slot: base node, then the arrival
+                                  rank and finally the sorted position */
   float *records;              /* [256*n_chunks][16] per-particle stencil record per launch slot */
+  float *sorted;               /* [256*n_chunks][16] the records in sorted order (gathered) */
   float *body_partial;         /* [n_blocks][max_bodies][16] per-block body-pose gradients */
   int32_t *range_count;        /* [(n_blocks + 1023) / 1024] flagged blocks per range */
   int32_t *counter;            /* [1] sorted-segment allocator; zero between substeps */
-  int32_t reserved[2];
 } ddm_det;                     /* 80 bytes */
 
 typedef struct ddm_batch {

@@ -39,6 +39,6 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.DdmScene) == 176
     assert ctypes.sizeof(_lib.DdmState) == 6 * 8
     assert ctypes.sizeof(_lib.DdmBatch) == 16 + 11 * 8 + 8
-    assert ctypes.sizeof(_lib.DdmDet) == 9 * 8 + 8
+    assert ctypes.sizeof(_lib.DdmDet) == 10 * 8
     hdr = open(HEADER).read()
     assert "} ddm_scene;                   /* 176 bytes */" in hdr
