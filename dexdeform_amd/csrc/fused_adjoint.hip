@@ -225,12 +225,13 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   // d.w}, node-velocity adjoint seed = b.xyz)
   auto contrib = [inv_dx](const StencilRecs &r, int s, int i, int j, float (*acc)[3]) {
     const float4 a = r.f0[s], b = r.f1[s], c = r.f2[s], d = r.f3[s];
-    const float wxy = r.w[9 * s + i] * r.w[9 * s + 3 + j];
+    const float wxy = bspline_wi(a.x, i) * bspline_wi(a.y, j);
+    const float wz[3] = {bspline_w<0>(a.z), bspline_w<1>(a.z), bspline_w<2>(a.z)};
     const float d0 = (float)i - a.x, d1 = (float)j - a.y;
     const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float w = wxy * r.w[9 * s + 6 + k];
+      const float w = wxy * wz[k];
       const float d2 = (float)k - a.z;
       const float xx = w * inv_dx * 4.f;
       acc[k][0] += w * b.x + (p0 + c.y * d2) * xx;

@@ -36,7 +36,7 @@ namespace {
 
 constexpr int kWG = 256;
 // LDS tile capacity in nodes. P2G keeps 4 floats per node and the G2P adjoint 3; with
-// the staged particle records both stay under 40 KB of LDS per workgroup, so four
+// the staged particle records both stay under 32 KB of LDS per workgroup, so five
 // workgroups (one wave per SIMD each) fit a CU.
 constexpr int kTileP2G = 704;
 constexpr int kTileG2P = 960;
@@ -100,21 +100,25 @@ __device__ __forceinline__ Tile slab(const Tile &T, int xs, int sx) {
 // Per-particle stencil data staged in LDS, in group order. f1..f3 hold a vector and a
 // row-major 3x3 matrix whose meaning is the kernel's (P2G: m v and the affine momentum
 // matrix; G2P adjoint: node-velocity adjoint seed and C adjoint).
+// The B-spline weights are recomputed from fx by the tasks (bspline_wi): fewer LDS reads
+// and 9 KB less LDS per workgroup than staging them.
 struct StencilRecs {
-  float4 f0[kWG];    // fx (3), scalar
-  float4 f1[kWG];    // vector (3), matrix[0]
-  float4 f2[kWG];    // matrix[1..4]
-  float4 f3[kWG];    // matrix[5..8]
-  float w[kWG * 9];  // B-spline weights w[axis][offset], 9 consecutive per particle
+  float4 f0[kWG];  // fx (3), scalar
+  float4 f1[kWG];  // vector (3), matrix[0]
+  float4 f2[kWG];  // matrix[1..4]
+  float4 f3[kWG];  // matrix[5..8]
 };
+// B-spline weight of offset i (0..2) for the fractional position f: stencil()'s expressions
+__device__ __forceinline__ float bspline_wi(float f, int i) {
+  const float a = i == 0 ? 1.5f - f : (i == 1 ? f - 1.f : f - 0.5f);
+  return i == 1 ? 0.75f - a * a : 0.5f * (a * a);
+}
 __device__ __forceinline__ void stage_record(StencilRecs &rc, int s, const Stencil &st, float scalar, f3 vec,
                                              const m33 &M) {
   rc.f0[s] = make_float4(st.fx[0], st.fx[1], st.fx[2], scalar);
   rc.f1[s] = make_float4(vec.x, vec.y, vec.z, M.a[0]);
   rc.f2[s] = make_float4(M.a[1], M.a[2], M.a[3], M.a[4]);
   rc.f3[s] = make_float4(M.a[5], M.a[6], M.a[7], M.a[8]);
-#pragma unroll
-  for (int e = 0; e < 9; ++e) rc.w[9 * s + e] = st.w[e / 3][e % 3];
 }
 __device__ __forceinline__ f3 rec_vec(const StencilRecs &rc, int s) {
   float4 b = rc.f1[s];

@@ -14,8 +14,9 @@ enum P2GMode { kForward = 0, kRecompute = 1, kForwardTaped = 2, kRecomputeUntape
 //                    (the taped adjoint reads them).
 // kRecomputeUntaped: kRecompute for the scenes whose snapshot was not restored (no
 //                    snapshot, or tape_status[1] set by k_restore); other chunks exit.
+// at most 96 VGPRs (a few bytes of spill): five waves per SIMD, as the LDS allows
 template <int MODE>
-__global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+__global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                              const ddm_state *__restrict__ next_tab) {
   __shared__ ScatterLDS<4, kTileP2G> sh;
   __shared__ StencilRecs rc;
@@ -63,12 +64,13 @@ __global__ __launch_bounds__(kWG) void k_p2g(ddm_batch B, const ddm_state *__res
   // d.z, d.w}, rows summed left to right as mvec does)
   auto contrib = [dx](const StencilRecs &r, int s, int i, int j, float (*acc)[4]) {
     const float4 a = r.f0[s], b = r.f1[s], c = r.f2[s], d = r.f3[s];
-    const float wxy = r.w[9 * s + i] * r.w[9 * s + 3 + j];
+    const float wxy = bspline_wi(a.x, i) * bspline_wi(a.y, j);
+    const float wz[3] = {bspline_w<0>(a.z), bspline_w<1>(a.z), bspline_w<2>(a.z)};
     const float d0 = ((float)i - a.x) * dx, d1 = ((float)j - a.y) * dx;
     const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float w = wxy * r.w[9 * s + 6 + k];
+      const float w = wxy * wz[k];
       const float d2 = ((float)k - a.z) * dx;
       acc[k][0] += a.w * w;
       acc[k][1] += (b.x + (p0 + c.y * d2)) * w;

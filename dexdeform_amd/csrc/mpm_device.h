@@ -262,6 +262,17 @@ DDM_HD f3 prim_grad_adj(const float *prim, const float *args, f3 p, f3 g_out, in
 // (8 sweeps), column sort by norm, Givens QR. The float-rounded constants of the
 // reference (svd.h:17-20) are kept so that the same rotations are chosen.
 namespace svd {
+// 1/sqrt(x) in FP64: the hardware estimate refined by three Newton steps (within a few
+// FP64 ulps of the correctly rounded value; the factors are returned in FP32, where this
+// difference from the reference's sqrt and division does not show)
+DDM_HD double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * (1.5 - h * (y * y));
+  y = y * (1.5 - h * (y * y));
+  y = y * (1.5 - h * (y * y));
+  return y;
+}
 constexpr double kGamma = (double)5.82842712474619f;
 constexpr double kCstar = (double)0.9238795325112867f;
 constexpr double kSstar = (double)0.3826834323650897f;
@@ -273,12 +284,13 @@ template <int X, int Y, int Z>
 DDM_HD void jacobi(double &s11, double &s21, double &s22, double &s31, double &s32, double &s33, double q[4]) {
   double ch = 2.0 * (s11 - s22), sh = s21;
   bool b = kGamma * sh * sh < ch * ch;
-  double w = 1.0 / sqrt(ch * ch + sh * sh);
+  double w = rsq64(ch * ch + sh * sh);
   ch = b ? w * ch : kCstar;
   sh = b ? w * sh : kSstar;
   double scale = ch * ch + sh * sh;
-  double a = (ch * ch - sh * sh) / scale;
-  double bb = (2.0 * sh * ch) / scale;
+  const double inv = 1.0 / scale;  // one division for both quotients
+  double a = (ch * ch - sh * sh) * inv;
+  double bb = (2.0 * sh * ch) * inv;
   double n11 = a * (a * s11 + bb * s21) + bb * (a * s21 + bb * s22);
   double n21 = a * (-bb * s11 + a * s21) + bb * (-bb * s21 + a * s22);
   double n22 = -bb * (-bb * s11 + a * s21) + a * (-bb * s21 + a * s22);
@@ -307,11 +319,11 @@ DDM_HD void cnswap(bool c, double &x, double &y) {
 }
 DDM_HD void qr_rot(double a1, double a2, double &ch, double &sh) {  // svd.h:248-262
   double r2 = a1 * a1 + a2 * a2;
-  double rho = r2 / sqrt(r2);
+  double rho = r2 * rsq64(r2);  // r2 / sqrt(r2); NaN for r2 = 0 as there
   sh = rho > kEps ? a2 : 0.0;
   ch = fabs(a1) + fmax(rho, kEps);
   cswap(a1 < 0.0, sh, ch);
-  double w = 1.0 / sqrt(ch * ch + sh * sh);
+  double w = rsq64(ch * ch + sh * sh);
   ch *= w;
   sh *= w;
 }
