@@ -55,8 +55,8 @@ class SceneSpec:
 
     def validate(self):
         gd = tuple(int(g) for g in self.grid_dim)
-        if len(gd) != 3 or any(g < 8 or g % 4 for g in gd):
-            raise ValueError(f"grid dims must be multiples of 4 and >= 8, got {gd}")
+        if len(gd) != 3 or any(g < 8 or g > 1024 or g % 4 for g in gd):
+            raise ValueError(f"grid dims must be multiples of 4 in [8, 1024], got {gd}")
         if self.n_bodies > _lib.DDM_MAX_BODIES:
             raise ValueError(f"at most {_lib.DDM_MAX_BODIES} bodies per scene, got {self.n_bodies}")
         for name in ("mass", "vol"):

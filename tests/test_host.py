@@ -151,3 +151,20 @@ def test_workloads_shapes_and_order():
     x = W.ellipsoid_particles(500, [0.5, 0.2, 0.5], [0.12, 0.08, 0.1], seed=3)
     r = ((x - [0.5, 0.2, 0.5]) / [0.12, 0.08, 0.1]) ** 2
     assert np.all(r.sum(1) <= 1 + 1e-5)
+
+
+def test_scene_validation_rejects_bad_grids_and_shapes():
+    """Grid dims must be multiples of 4 in [8, 1024] (stencil bases are packed in 10 bits
+    per axis), at most DDM_MAX_BODIES bodies, per-particle arrays of the right shape."""
+    import pytest
+    from dexdeform_amd.mpm import workloads as W
+    sc = W.plain_scene(n_particles=100)
+    sc.spec.validate()
+    for gd in ((64, 64, 62), (4, 64, 64), (1028, 64, 64)):
+        sc.spec.grid_dim = gd
+        with pytest.raises(ValueError):
+            sc.spec.validate()
+    sc.spec.grid_dim = (64, 64, 64)
+    sc.spec.mass = sc.spec.mass[:-1]
+    with pytest.raises(ValueError):
+        sc.spec.validate()
