@@ -45,6 +45,7 @@ import torch
 
 from .. import _lib
 from .engine import CHUNK, SceneSpec, chunk_table
+from .workloads import spatial_order, spatially_ordered
 
 
 @dataclass
@@ -266,7 +267,13 @@ class _Active:
         if T % self.K:
             raise ValueError("checkpoint_every must divide the horizon")
         f = lambda a: _to_device(a, dev)
-        self.mass, self.vol, self.mly = f(sp.mass), f(sp.vol), f(np.asarray(sp.mu_lam_yield).reshape(-1, 3))
+        # particles not in spatial order (4^3 block, then cell) are sorted for the kernels;
+        # results are returned in the request's order
+        self.order = None if spatially_ordered(req.x0, sp.dx) else spatial_order(req.x0, sp.dx)
+        srt = (lambda a: np.asarray(a)[self.order]) if self.order is not None else (lambda a: a)
+        self.inv = (torch.as_tensor(np.argsort(self.order), device=dev) if self.order is not None else None)
+        self.mass, self.vol, self.mly = (f(srt(sp.mass)), f(srt(sp.vol)),
+                                         f(srt(np.asarray(sp.mu_lam_yield).reshape(-1, 3))))
         self.prim = f(np.asarray(sp.body_prim).reshape(-1, 4)) if nb else None
         self.args = f(np.asarray(sp.body_args).reshape(-1, 4)) if nb else None
         # taped adjoint: per substep of a segment, the SVD temporaries and a grid snapshot
@@ -293,14 +300,14 @@ class _Active:
         self.seg = sets[n_ckpt:n_ckpt + self.K + 1]
         self.g = sets[n_ckpt + self.K + 1:]
         s0 = self.seg[0]
-        s0.x.copy_(f(req.x0))
+        s0.x.copy_(f(srt(req.x0)))
         s0.v.zero_()
         s0.C.zero_()
         s0.F.zero_()
         s0.F.diagonal(dim1=1, dim2=2).fill_(1.0)
         if not self.full:
             self.ckpt[0].copy_(self.seg[0])
-        self.target = f(req.target)
+        self.target = f(srt(req.target))
         self.record = np.zeros(1, dtype=_lib.SCENE_DTYPE)[0]
         r = self.record
         r["p_count"] = n
@@ -602,8 +609,11 @@ class RolloutServer:
                 a.done_event.synchronize()
                 fin = self._now()
                 g0 = a.g[a.final_grad_slot]
-                self.results[a.req.rid] = Result(a.req.rid, float(a.loss_host.item()), a.x_final, a.pos_grad,
-                                                 a.rot_grad, g0.x.clone(), a.req.arrival, a.admitted, fin)
+                x_final, x0_grad = a.x_final, g0.x.clone()
+                if a.inv is not None:  # back to the request's particle order
+                    x_final, x0_grad = x_final[a.inv], x0_grad[a.inv]
+                self.results[a.req.rid] = Result(a.req.rid, float(a.loss_host.item()), x_final, a.pos_grad,
+                                                 a.rot_grad, x0_grad, a.req.arrival, a.admitted, fin)
                 self.used_bytes -= request_bytes(a.req)
                 self.arena.release(a.blk_begin, a.n_blocks)
             else:

@@ -19,12 +19,23 @@ CAPSULE, BOX = 1, 0
 SOFTNESS = 666.0  # every primitive gets softness 666 (mpm/cuda_env.py:88)
 
 
-def spatial_order(x: np.ndarray, dx: float) -> np.ndarray:
-    """Permutation that groups particles by 4^3 cell block, then by cell."""
-    c = np.floor(x / dx).astype(np.int64)
+def spatial_key(x: np.ndarray, dx: float) -> np.ndarray:
+    """Sort key of each particle: its 4^3 cell block, then its cell in the block."""
+    c = np.floor(np.asarray(x) / dx).astype(np.int64)
     b = c >> 2
-    key = ((b[:, 0] * 1024 + b[:, 1]) * 1024 + b[:, 2]) * 64 + ((c[:, 0] & 3) * 16 + (c[:, 1] & 3) * 4 + (c[:, 2] & 3))
-    return np.argsort(key, kind="stable")
+    return ((b[:, 0] * 1024 + b[:, 1]) * 1024 + b[:, 2]) * 64 + ((c[:, 0] & 3) * 16 + (c[:, 1] & 3) * 4 + (c[:, 2] & 3))
+
+
+def spatial_order(x: np.ndarray, dx: float) -> np.ndarray:
+    """Permutation that groups particles by 4^3 cell block, then by cell. The particle
+    kernels stage each chunk of 256 consecutive particles in an LDS tile around the
+    chunk, so spatially ordered particles are several times faster than random order."""
+    return np.argsort(spatial_key(x, dx), kind="stable")
+
+
+def spatially_ordered(x: np.ndarray, dx: float) -> bool:
+    k = spatial_key(x, dx)
+    return bool(np.all(k[1:] >= k[:-1]))
 
 
 def box_particles(n: int, lo, hi, seed: int, dx: float = 1 / 64) -> np.ndarray:

@@ -9,6 +9,8 @@
 Tolerance: 1e-4 relative for states and losses, 1e-3 for gradients (FP32 atomic order
 differs between runs).
 """
+import copy
+
 import numpy as np
 import pytest
 import torch
@@ -153,6 +155,25 @@ def test_server_graph_ticks_match_eager_ticks():
         e0 = rel_err(b.x0_grad.cpu().numpy(), a.x0_grad.cpu().numpy())
         print(f"request {rid}: graph vs eager: loss {el:.1e} pose grad {eg:.1e} x0 grad {e0:.1e}")
         assert el <= 1e-5 and eg <= 1e-3 and e0 <= 1e-3
+
+
+def test_server_sorts_particles_given_in_random_order():
+    """A request whose particles come in random order (as the reference's samplers make
+    them) is sorted at admission and answered in its own order: the same loss, final
+    positions and gradients as the spatially ordered request."""
+    r = _requests()[1]
+    perm = np.random.default_rng(8).permutation(r.n)
+    sp = copy.deepcopy(r.spec)
+    sp.mass, sp.vol, sp.mu_lam_yield = sp.mass[perm], sp.vol[perm], np.asarray(sp.mu_lam_yield)[perm]
+    rs = Request(r.rid, sp, r.x0[perm], r.pose_pos, r.pose_rot, r.target[perm], checkpoint_every=r.checkpoint_every)
+    a = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30).run([r])[r.rid]
+    b = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30).run([rs])[r.rid]
+    el = abs(a.loss - b.loss) / abs(a.loss)
+    ex = rel_err(b.x_final.cpu().numpy(), a.x_final.cpu().numpy()[perm])
+    e0 = rel_err(b.x0_grad.cpu().numpy(), a.x0_grad.cpu().numpy()[perm])
+    eg = rel_err(b.pos_grad.cpu().numpy(), a.pos_grad.cpu().numpy())
+    print(f"random-order request vs ordered: loss {el:.1e} x {ex:.1e} x0 grad {e0:.1e} pose grad {eg:.1e}")
+    assert el <= 1e-5 and ex <= 1e-5 and e0 <= 1e-3 and eg <= 1e-3
 
 
 def test_server_matches_direct_rollout():
