@@ -129,6 +129,64 @@ def serve_benchmark(dev, rates, seconds, seed, graphs=True):
     return out
 
 
+def named_rollouts(dev, steps, large=True):
+    """The single-scene workloads, each one forward + backward rollout through the
+    single-scene path (taped adjoint): the plain 10k-particle scene (50 substeps, two
+    capsules with constant velocities), the 50k-particle hand-contact scene (22 links,
+    200 substeps, checkpoints every 10, seeded joint-velocity actions) and the
+    1M-particle 128^3 scene (500 substeps, checkpoints every 25). The first two are
+    replayed from HIP graphs. Reports particle-substeps/s (forward + adjoint counted
+    once) and ms per rollout."""
+    import torch
+    from dexdeform_amd.mpm import workloads as W
+    from dexdeform_amd.mpm.engine import Engine
+    from dexdeform_amd.mpm.rollout import Rollout
+    out = {}
+
+    def run(name, sc, T, K, poses, graph):
+        eng = Engine([sc.spec], device=dev)
+        ro = Rollout(eng, T, checkpoint_every=K, tape=True)
+        ro.set_initial(torch.tensor(sc.x0, device=dev))
+        ro.set_poses(*poses)
+        target = torch.tensor(W.plain_target(sc.x0), device=dev)
+        if graph:
+            d = ro.forward().x - target
+            seed = (2.0 / sc.n) * d
+            ro.capture(seed)
+            step = ro.replay
+        else:
+            def step():
+                d = ro.forward().x - target
+                ro.backward(x_grad=(2.0 / sc.n) * d)
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        out[name] = {"particles": sc.n, "substeps": T, "checkpoint_every": K, "graph": graph,
+                     "ms_per_rollout": sec * 1e3, "particle_substeps_per_s": sc.n * T / sec}
+        log(f"{name}: {sc.n * T / sec:.3e} particle-substeps/s, {sec * 1e3:.2f} ms per forward + backward rollout")
+        del ro, eng
+        torch.cuda.empty_cache()
+
+    sc = W.plain_scene(n_particles=10000)
+    T = 50
+    vel = np.random.Generator(np.random.PCG64(1)).uniform(-0.5, 0.5, size=(sc.nb, 3)).astype(np.float32)
+    t = np.arange(T + 1, dtype=np.float32)[:, None, None] * sc.spec.dt
+    run("plain_10k", sc, T, None, (torch.tensor(sc.body_pos[None] + t * vel[None], device=dev),
+                                    torch.tensor(np.repeat(sc.body_rot[None], T + 1, 0), device=dev)), True)
+    sc, tree, q0 = W.hand_scene(50000)
+    P, R = tree.to(device=dev).trajectory(q0.to(dev), torch.tensor(W.hand_actions(20), device=dev), 10)
+    run("hand_50k", sc, 200, 10, (P.detach(), R.detach()), True)
+    if large:
+        sc, tree, q0 = W.large_scene(1_000_000)
+        P, R = tree.to(device=dev).trajectory(q0.to(dev), torch.tensor(W.hand_actions(50), device=dev), 10)
+        run("large_1m", sc, 500, 25, (P.detach(), R.detach()), False)
+    return out
+
+
 def run_step(ro, gx):
     ro.forward()
     ro.backward(x_grad=gx)
@@ -259,6 +317,8 @@ def main():
                     help="skip the extra measurement of the deterministic sorted-reduction mode")
     ap.add_argument("--serve-rates", default="10,50,100,200", help="request arrival rates (req/s) to sweep")
     ap.add_argument("--serve-eager", action="store_true", help="serve without capturing ticks in HIP graphs")
+    ap.add_argument("--no-named", action="store_true",
+                    help="skip the single-scene plain 10k / hand 50k / large 1M rollouts")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival window per rate: rate x seconds requests (at least 24)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -378,6 +438,8 @@ def main():
         plain_value = tot_ps / tot_g
         plain_eager_value = tot_ps / tot_e
 
+    named = None if args.no_named else named_rollouts(dev, max(1, min(args.steps, 3)))
+
     serve = None
     if not args.no_serve:
         serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_seconds, args.seed,
@@ -418,6 +480,7 @@ def main():
                      "traffic": (TRAFFIC_BYTES[("taped_" if tape else "") + dom] * n_total
                                  if (args.scenes, args.seed) == (64, 7) and
                                  ("taped_" if tape else "") + dom in TRAFFIC_BYTES else None)},
+        "single_scene_rollouts": named,
         "serve": serve,
         "cpu_baseline": cpu,
     }
