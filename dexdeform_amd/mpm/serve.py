@@ -44,7 +44,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .engine import CHUNK, SceneSpec, chunk_table
+from .engine import CHUNK, SceneSpec, chunk_size_for, chunk_table
 from .workloads import spatial_order, spatially_ordered
 
 
@@ -323,7 +323,9 @@ class _Active:
         r["body_prim"] = self.prim.data_ptr() if nb else 0
         r["body_args"] = self.args.data_ptr() if nb else 0
         self._temps_of(r, 0)  # untaped forward substeps write no temporaries; any slot will do
-        self.chunks = chunk_table([n]).astype(np.int32)  # (0, first, end, 0) rows; column 0 = scene slot
+        # (0, first, end, 0) rows; column 0 = scene slot. Smaller chunks for small requests,
+        # which often run alone at low load
+        self.chunks = chunk_table([n], chunk=chunk_size_for(n)).astype(np.int32)
         self.ops = self._schedule()
         self._addresses()
         self.pc = 0

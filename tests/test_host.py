@@ -168,3 +168,17 @@ def test_scene_validation_rejects_bad_grids_and_shapes():
     sc.spec.mass = sc.spec.mass[:-1]
     with pytest.raises(ValueError):
         sc.spec.validate()
+
+
+def test_chunk_size_for_small_and_large_launches():
+    from dexdeform_amd.mpm.engine import chunk_size_for, chunk_table
+    assert chunk_size_for(10_000) == 64
+    assert chunk_size_for(30_000) == 128
+    assert chunk_size_for(50_000) == 256
+    assert chunk_size_for(2_000_000) == 256
+    for c in (64, 128, 256):
+        tab = chunk_table([1000, 0, 300], chunk=c)
+        assert (tab[:, 2] - tab[:, 1]).max() <= c
+        for s, n in ((0, 1000), (2, 300)):
+            rows = tab[tab[:, 0] == s]
+            assert rows[0, 1] == 0 and rows[-1, 2] == n and np.array_equal(rows[1:, 1], rows[:-1, 2])
