@@ -167,7 +167,12 @@ class MPMSimulator:
     def __init__(self, n_bodies, cfg=None, ground_friction=0., gravity=(0., -1., 0.), n_particles=20000,
                  dx=1. / 64, dt=0.0001, grid_size=(1., 1., 1.), max_steps=30, substeps=20, yield_stress=30.,
                  vol=(1. / 64 / 2) ** 2, mass=(1. / 64 / 2) ** 2, E=5000., nu=0.2, device="cuda:0",
-                 adjoint="reference"):
+                 adjoint="reference", sort_particles=True):
+        """sort_particles: particles given in random order (as the reference's samplers
+        produce them) are kept in spatial order inside the simulator (the particle
+        kernels are 2-4x faster on it); every per-particle input and output of the
+        public methods stays in the caller's order. ``states[i]`` buffers then hold the
+        internal order."""
         kw = dict(ground_friction=ground_friction, gravity=gravity, n_particles=n_particles, dx=dx, dt=dt,
                   grid_size=grid_size, max_steps=max_steps, substeps=substeps, yield_stress=yield_stress, vol=vol,
                   mass=mass, E=E, nu=nu)
@@ -212,6 +217,10 @@ class MPMSimulator:
         self.body_args = DeviceArray(self.engine.body_args)
         self._grid_lower = torch.zeros(1, 3, dtype=torch.int32, device=self.device)
         self.object_id = None
+        # particle order: internal row i holds the caller's particle _perm[i] (None: same order)
+        self.sort_particles = bool(sort_particles)
+        self._perm = self._inv = self._perm_np = self._inv_np = None
+        self._caller_particle_arrays = {}  # caller-order copies re-uploaded when the order changes
         self.torch_scale = None
         self.action_scales = None
         self.cur = 0
@@ -232,6 +241,39 @@ class MPMSimulator:
     def _stream(self):
         return _lib.stream_ptr()
 
+    # ------------------------------------------------------------------ particle order
+    def _to_internal(self, a):
+        """Rows in the caller's particle order -> the internal order."""
+        if self._perm is None:
+            return a
+        if isinstance(a, torch.Tensor):
+            return a[self._perm.to(a.device)]
+        return np.asarray(a)[self._perm_np]
+
+    def _to_caller(self, a):
+        """Rows in the internal particle order -> the caller's order."""
+        if self._perm is None:
+            return a
+        if isinstance(a, torch.Tensor):
+            return a[self._inv.to(a.device)]
+        return np.asarray(a)[self._inv_np]
+
+    def _choose_order(self, x):
+        """Keep the caller's order unless it is far from spatial (more than 10 % of
+        neighbouring particles out of block/cell order, e.g. randomly sampled)."""
+        from .workloads import spatial_key, spatial_order
+        x = np.asarray(x, np.float32)
+        k = spatial_key(x, self.dx)
+        if not self.sort_particles or len(k) < 2 or np.mean(k[1:] < k[:-1]) <= 0.1:
+            self._perm = self._inv = self._perm_np = self._inv_np = None
+        else:
+            self._perm_np = spatial_order(x, self.dx)
+            self._inv_np = np.argsort(self._perm_np)
+            self._perm = torch.as_tensor(self._perm_np, device=self.device)
+            self._inv = torch.as_tensor(self._inv_np, device=self.device)
+        for name, a in self._caller_particle_arrays.items():  # per-particle inputs in the new order
+            getattr(self, name).upload(self._to_internal(a))
+
     def _set_n(self, n):
         n = int(n)
         if n > self.max_particles:
@@ -246,29 +288,36 @@ class MPMSimulator:
 
     def get_state(self, index):
         s = self.states[index].get_state(self.n_particles)
-        return tuple(s[:4]) + tuple(np.r_[a, b] for a, b in zip(s[4], s[5]))
+        return tuple(self._to_caller(a) for a in s[:4]) + tuple(np.r_[a, b] for a, b in zip(s[4], s[5]))
 
     def set_state(self, index, state):
+        """(x, v, F, C, pose per body). At index 0, or for a new particle count, the
+        particle order is chosen from x (see sort_particles); otherwise the current order
+        applies (e.g. to a state returned by get_state)."""
         state = [np.float32(i) for i in state]
+        if index == 0 or len(state[0]) != self.n_particles or (self._perm is None and self.sort_particles):
+            self._choose_order(state[0])
         self._set_n(len(state[0]))
         pos = np.float32([i[:3] for i in state[4:]]).reshape(-1, 3)
         rot = np.float32([i[3:] for i in state[4:]]).reshape(-1, 4)
-        self.states[index].set_state(state[:4] + [pos, rot])
+        self.states[index].set_state([self._to_internal(a) for a in state[:4]] + [pos, rot])
         self.cur = index
 
     def set_object_id(self, object_id):
         self.object_id = object_id
         assert len(object_id) == self.n_particles, f"{len(object_id)} {self.n_particles}"
-        self.particle_ids.upload(np.int32(object_id))
+        self._caller_particle_arrays["particle_ids"] = np.int32(object_id)
+        self.particle_ids.upload(self._to_internal(np.int32(object_id)))
 
     def get_object_id(self, device):
         assert device == "numpy"
         return self.object_id
 
     def set_color(self, inp):
-        color = self.particle_color.download(self.n_particles)
+        color = self._to_caller(self.particle_color.download(self.n_particles))
         color[:] = inp
-        self.particle_color.upload(color)
+        self._caller_particle_arrays["particle_color"] = color
+        self.particle_color.upload(self._to_internal(color))
 
     def set_softness(self, softness):
         pp = self.body_type_mu_softness_round.download()
@@ -279,10 +328,10 @@ class MPMSimulator:
         return self.body_type_mu_softness_round.download()[:, 2]
 
     def get_x(self, index, device="numpy"):
-        return self.states[index].x.download(self.n_particles, device=device)
+        return self._to_caller(self.states[index].x.download(self.n_particles, device=device))
 
     def get_v(self, index, device="numpy"):
-        return self.states[index].v.download(self.n_particles, device=device)
+        return self._to_caller(self.states[index].v.download(self.n_particles, device=device))
 
     def get_tool_state(self, index=0, device=None):
         pos = self.states[index].body_pos.download()
@@ -311,9 +360,9 @@ class MPMSimulator:
         assert vol.shape == (self.n_particles,)
         assert mass.shape == (self.n_particles,)
         assert mu_lam_yield.shape == (self.n_particles, 3)
-        self.particle_vol.upload(vol)
-        self.particle_mass.upload(mass)
-        self.particle_mu_lam_yield.upload(mu_lam_yield)
+        for name, a in (("particle_vol", vol), ("particle_mass", mass), ("particle_mu_lam_yield", mu_lam_yield)):
+            self._caller_particle_arrays[name] = np.float32(a)
+            getattr(self, name).upload(self._to_internal(np.float32(a)))
 
     def init_bodies(self, types, softness, mu, round, args, action_scales, pos=None, rot=None):
         assert len(mu) == len(args) == self.n_bodies
@@ -337,14 +386,15 @@ class MPMSimulator:
         need_grad = grad is not None
         if need_grad:
             assert tuple(grad.shape) == (n, nb)
-            self.dists_grad.tensor[: n * nb].copy_(torch.as_tensor(grad, dtype=torch.float32).reshape(-1))
+            g = self._to_internal(torch.as_tensor(grad, dtype=torch.float32, device=self.device))
+            self.dists_grad.tensor[: n * nb].copy_(g.reshape(-1))
         _lib.check(self.lib.compute_dist(
             state.x.data_ptr, state.body_pos.data_ptr, state.body_rot.data_ptr,
             self.body_type_mu_softness_round.data_ptr, self.body_args.data_ptr, self.dists.data_ptr, nb,
             state.x_grad.data_ptr, state.body_pos_grad.data_ptr, state.body_rot_grad.data_ptr,
             self.dists_grad.data_ptr, int(need_grad), n, self._stream()), "compute_dist")
         if not need_grad:
-            return self.dists.download(n * nb, device=device).reshape(-1, nb)
+            return self._to_caller(self.dists.download(n * nb, device=device).reshape(-1, nb))
 
     def compute_grid_mass(self, f, id=-1, device="numpy", backward_grad=None):
         need_grad = backward_grad is not None
@@ -355,6 +405,8 @@ class MPMSimulator:
         else:
             x = torch.as_tensor(f, dtype=torch.float32) if not isinstance(f, torch.Tensor) else f.detach().float()
             assert x.shape[-1] == 3 and id == -1
+            if len(x) == self.n_particles:  # positions of the scene's particles, in the caller's order
+                x = self._to_internal(x.to(self.device))
             state = self.states[-1]
             state.x.upload(x)
             n = len(x)

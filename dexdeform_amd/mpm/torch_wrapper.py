@@ -58,8 +58,8 @@ class GradModel:
         f = s * sim.substeps
         st = sim.states[f]
         n = sim.n_particles
-        x = st.x.tensor[:n].detach().clone()
-        v = st.v.tensor[:n].detach().clone()
+        x = sim._to_caller(st.x.tensor[:n].detach().clone())
+        v = sim._to_caller(st.v.tensor[:n].detach().clone())
         c = torch.cat((st.body_pos.tensor, st.body_rot.tensor), 1).detach().clone()
         outs = []
         if sim.n_bodies:
@@ -88,7 +88,7 @@ class GradModel:
             if nb:
                 sim.get_dists(f, particle_grad[:, -nb:].contiguous())
                 particle_grad = particle_grad[:, :-nb]
-            pg = particle_grad.reshape(-1, self.dim * 2)
+            pg = sim._to_internal(particle_grad.reshape(-1, self.dim * 2))
             st.x_grad.tensor[:n] += pg[:, :3]
             st.v_grad.tensor[:n] += pg[:, 3:]
         if tool_grad is not None and nb:
