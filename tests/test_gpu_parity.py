@@ -425,3 +425,58 @@ def test_simulator_keeps_random_order_particles_sorted_inside(plain):
     ex, ed, ec, eg = rel_err(b[0], a[0]), rel_err(b[1], a[1]), rel_err(b[2], a[2]), rel_err(b[3], a[3])
     print(f"random-order particles vs ordered: x {ex:.1e} dists {ed:.1e} C {ec:.1e} action grad {eg:.1e}")
     assert ex <= 1e-5 and ed <= 1e-5 and ec <= 1e-4 and eg <= 1e-3
+
+
+@pytest.mark.parametrize("gf,mode", [(0.5, 0), (0.5, 1), (100.0, 0)])
+def test_box_capsule_and_ground_friction_substep(gf, mode):
+    """Grid update and its adjoint with a rotated box primitive, a capsule, and ground
+    friction (0.5: Coulomb friction on the floor; 100: the sticky floor, >= 99), against
+    the oracle: particles fall onto the floor with a sideways velocity, under the box
+    (one forward substep <= 1e-5 per array; one adjoint substep <= max(1e-3, 2 x the
+    oracle's FP32-vs-FP64 discrepancy) per array)."""
+    dx = 1 / 64
+    x0 = W.box_particles(4000, [0.42, 0.052, 0.42], [0.58, 0.12, 0.58], 11, dx)
+    prim = np.float32([[W.BOX, 0.9, 666.0, 0.0], [W.CAPSULE, 0.9, 666.0, 0.0]])
+    args = np.float32([[0.05, 0.02, 0.04, 0.0], [0.02, 0.06, 0.0, 0.0]])
+    pos = np.float32([[0.50, 0.135, 0.50], [0.45, 0.13, 0.55]])
+    ang = 0.3
+    rot = np.float32([[np.cos(ang / 2), 0.0, np.sin(ang / 2), 0.0], [np.cos(0.7), np.sin(0.7), 0.0, 0.0]])
+    sc = W.make_scene(x0, 64, dx, 5e-5, prim, args, pos, rot, ground_friction=gf, adjoint_mode=mode)
+    rng = np.random.default_rng(12)
+    st = initial_state(sc, rng, v_scale=0.3, f_scale=0.02, c_scale=0.5)
+    st.v[:, 0] += 0.8
+    st.v[:, 1] -= 1.5
+    p1 = pos + np.float32([0.0, -0.5, 0.2]) * sc.spec.dt
+    osim = O.OracleSim(oracle_scene(sc), np.float32, adjoint="exact" if mode else "reference")
+    nx = osim.substep(st, p1, rot)
+    eng = Engine([sc.spec], device=_dev(), with_grad=True)
+    cur, nxt, gcur, gnxt = eng.new_state(), eng.new_state(), eng.new_grad(), eng.new_grad()
+    _to_buf(st, cur)
+    nxt.body_pos.copy_(torch.from_numpy(p1))
+    nxt.body_rot.copy_(torch.from_numpy(rot))
+    eng.forward(cur, nxt)
+    torch.cuda.synchronize()
+    for name, g, r in zip("xvFC", _np(nxt), (nx.x, nx.v, nx.F, nx.C)):
+        e = rel_err(g, r)
+        print(f"gf={gf} forward {name}: rel err {e:.2e}")
+        assert e <= 1e-5, (name, e)
+    on_floor = np.mean(nx.x[:, 1] <= 3.5 * dx)
+    print(f"particles at the floor after the substep: {on_floor:.1%}")
+    _to_buf(nx, nxt)
+    g_next = _random_grads(sc.n, sc.nb, seed=13)
+    _to_buf(O.State(g_next.x, g_next.v, g_next.F, g_next.C, g_next.body_pos, g_next.body_rot), gnxt)
+    eng.backward(cur, nxt, gcur, gnxt)
+    torch.cuda.synchronize()
+    ref = _fp32_noise_bound(sc, st, nx, g_next, mode)
+    g32, gn32 = ref[np.float32]
+    g64, gn64 = ref[np.float64]
+    got = _np(gcur) + [gcur.body_pos.cpu().numpy(), gcur.body_rot.cpu().numpy(), gnxt.body_pos.cpu().numpy(),
+                       gnxt.body_rot.cpu().numpy()]
+    r32 = [g32.x, g32.v, g32.F, g32.C, g32.body_pos, g32.body_rot, gn32.body_pos, gn32.body_rot]
+    r64 = [g64.x, g64.v, g64.F, g64.C, g64.body_pos, g64.body_rot, gn64.body_pos, gn64.body_rot]
+    for name, g, a, b in zip(["x", "v", "F", "C", "body_pos", "body_rot", "next_body_pos", "next_body_rot"], got,
+                             r32, r64):
+        e, noise = rel_err(g, a), rel_err(a, b)
+        tol = max(1e-3, 2 * noise)
+        print(f"gf={gf} mode={mode} adjoint {name}: rel err {e:.2e} (fp32 noise {noise:.2e}, tol {tol:.1e})")
+        assert e <= tol, (name, e)
