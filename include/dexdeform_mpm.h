@@ -201,8 +201,9 @@ int ddm_substep_forward(const ddm_batch *b, const ddm_state *cur, const ddm_stat
 
 /* Adjoint of one substep of every scene. Reads cur, next (state and poses) and the
  * next_grad x, v, F, C; ADDS into the cur_grad x, v, F, C, body_pos, body_rot and into
- * next_grad body_pos, body_rot. All four are DEVICE arrays of n_scenes records.
- * Replaces MPMSimulator.substep_grad. */
+ * next_grad body_pos, body_rot (with b->options & DDM_GRAD_OVERWRITE it WRITES the cur_grad
+ * x, v, F, C instead; pose gradients are still added). All four are DEVICE arrays of
+ * n_scenes records. Replaces MPMSimulator.substep_grad. */
 int ddm_substep_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                          const ddm_state *next_grad, hipStream_t stream);
 
