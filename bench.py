@@ -187,6 +187,60 @@ def named_rollouts(dev, steps, large=True):
     return out
 
 
+def trajectory_optimisation(dev, iters=2):
+    """One trajectory-optimisation iteration through the drop-in MPMSimulator + GradModel
+    (eager, one Python call per substep, as a reference user runs it): the lift-box
+    configuration -- 10,000 particles sampled in random order in a 0.09 box, dx 1/64,
+    dt 5e-5, 40 substeps per env step, 50 env steps, gravity (0,-2,0), ground friction
+    0.3 -- with 19 capsule primitives driven by per-step velocity actions; L2 loss of the
+    final positions to a target, backward, one Adam step. Seconds per iteration."""
+    import torch
+    from dexdeform_amd.mpm import workloads as W
+    from dexdeform_amd.mpm.simulator import MPMSimulator
+    from dexdeform_amd.mpm.torch_wrapper import GradModel
+    n, nb, S, E = 10000, 19, 40, 50
+    rng = np.random.Generator(np.random.PCG64(21))
+    x0 = rng.uniform([0.445, 0.175, 0.405], [0.535, 0.265, 0.495], size=(n, 3)).astype(np.float32)
+    pos, rot, args = W.random_capsules(rng, nb, [0.49, 0.31, 0.45], spread=0.05, r_range=(0.008, 0.015),
+                                       h_range=(0.02, 0.05))
+    sim = MPMSimulator(n_bodies=nb, n_particles=n, dx=1 / 64, dt=5e-5, substeps=S, max_steps=E * S + 1,
+                       gravity=(0., -2., 0.), ground_friction=0.3, yield_stress=50., E=5e3, nu=0.2,
+                       vol=(1 / 128) ** 2, mass=(1 / 128) ** 2, device=dev)
+    sim.init_bodies(types=[W.CAPSULE] * nb, softness=[666.] * nb, mu=[0.9] * nb, round=[0.] * nb, args=args,
+                    action_scales=np.tile(np.float32([S * 5e-5] * 3 + [0.0] * 3), (nb, 1)))
+    state = (x0, np.zeros_like(x0), np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)), np.zeros((n, 3, 3), np.float32))
+    state += tuple(np.r_[p, r] for p, r in zip(pos, rot))
+    target = torch.tensor(x0 + np.float32([0, 0.05, 0]), device=dev)
+    acts = torch.zeros(E, nb * 6, device=dev, requires_grad=True)
+    with torch.no_grad():
+        acts[:, 1::6] = -0.3  # press down
+    opt = torch.optim.Adam([acts], lr=0.01)
+    model = GradModel(sim, return_grid=())
+
+    def iteration():
+        sim.set_state(0, state)
+        model.zero_grad()
+        opt.zero_grad()
+        obs = model.get_obs(0, dev)
+        for e in range(E):
+            obs = model.forward(e, acts[e], *obs)
+        loss = ((obs[0][:, :3] - target) ** 2).sum(-1).mean()
+        loss.backward()
+        opt.step()
+        return float(loss.item())
+
+    iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        loss = iteration()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / iters
+    log(f"trajectory optimisation (drop-in simulator, lift-box shape): {sec:.3f} s per iteration, loss {loss:.5f}")
+    return {"seconds_per_iteration": sec, "particle_substeps_per_s": n * E * S / sec, "substeps": E * S,
+            "particles": n, "bodies": nb}
+
+
 def run_step(ro, gx):
     ro.forward()
     ro.backward(x_grad=gx)
@@ -439,6 +493,7 @@ def main():
         plain_eager_value = tot_ps / tot_e
 
     named = None if args.no_named else named_rollouts(dev, max(1, min(args.steps, 3)))
+    tutorial = None if args.no_named else trajectory_optimisation(dev)
 
     serve = None
     if not args.no_serve:
@@ -481,6 +536,7 @@ def main():
                                  if (args.scenes, args.seed) == (64, 7) and
                                  ("taped_" if tape else "") + dom in TRAFFIC_BYTES else None)},
         "single_scene_rollouts": named,
+        "trajectory_optimisation": tutorial,
         "serve": serve,
         "cpu_baseline": cpu,
     }

@@ -17,7 +17,15 @@ from torch.autograd import Function
 
 
 class GradModel:
-    def __init__(self, env, return_grid=(-1,), return_svd=False):
+    def __init__(self, env, return_grid=(-1,), return_svd=False, graphs=True):
+        """graphs: replay each environment step's substeps (forward, and the adjoint
+        substeps of backward) from a HIP graph captured on first use, instead of one
+        Python call per substep; the results are the same. Graphs are keyed by the step's
+        state buffers and the simulator's launch tables, and recaptured when those change."""
+        self.graphs = bool(graphs)
+        self._graphs = {}
+        self._pose_in = None
+        self._pose_grad_out = None
         self.env = env
         self.sim = env.simulator if hasattr(env, "simulator") else env
         self.dim = 3
@@ -96,6 +104,90 @@ class GradModel:
             st.body_pos_grad.tensor += tg[:, :3]
             st.body_rot_grad.tensor += tg[:, 3:]
 
+    # ------------------------------------------------------------------ HIP graphs
+    def _graph_run(self, kind, f, fn):
+        """Run fn() -- launches and device copies on fixed buffers for the environment
+        step starting at substep f -- eagerly the first time (which also fills the launch
+        table caches), then from a captured graph."""
+        sim = self.sim
+        if not self.graphs or not sim.device.type == "cuda":
+            fn()
+            return
+        key = (kind, f, id(sim.engine._batch)) + tuple(id(s) for s in sim.states[f:f + self.substeps + 1])
+        g = self._graphs.get(key)
+        if g is not None:
+            g.replay()
+            return
+        fn()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            fn()
+        self._graphs[key] = g
+
+    def _pose_buffers(self, pos):
+        S, nb = self.substeps, self.sim.n_bodies
+        if self._pose_in is None or self._pose_in[0].device != pos.device:
+            z = lambda *sh: torch.zeros(*sh, dtype=torch.float32, device=self.sim.device)
+            self._pose_in = (z(S, nb, 3), z(S, nb, 4))
+            self._pose_grad_out = (z(S, nb, 3), z(S, nb, 4))
+        return self._pose_in
+
+    def _substeps_forward(self, f, pos, rot):
+        """Substeps f .. f+S-1 with the given poses. Gradient buffers: the adjoint substeps
+        of backward write (DDM_GRAD_OVERWRITE) the particle gradients of states f .. f+S-1,
+        so forward clears only their pose gradients (accumulated by the contact adjoint)
+        and all gradients of the environment step's last state, which may be the rollout's
+        end and then receives only observation gradients (the reference clears every
+        gradient of every substep, mpm/simulator.py:570-571: same values)."""
+        sim, S = self.sim, self.substeps
+        states = sim.states[f + 1:f + S + 1]
+        if not self.graphs:
+            for i in range(S):
+                sim.set_pose(states[i], pos[i], rot[i])
+                sim.substep(f + i, clear_grad=True)
+            return
+        pin, rin = self._pose_buffers(pos)
+        pin.copy_(pos.reshape(pin.shape))
+        rin.copy_(rot.reshape(rin.shape))
+        dst = [s.body_pos.tensor for s in states] + [s.body_rot.tensor for s in states]
+        src = list(pin) + list(rin)
+        clear = ([s.grad.body_pos for s in states[:-1]] + [s.grad.body_rot for s in states[:-1]] +
+                 list(states[-1].grad.tensors()))
+
+        def run():
+            torch._foreach_copy_(dst, src)  # one launch for the environment step's poses
+            torch._foreach_zero_(clear)
+            for i in range(S):
+                sim.engine.forward(sim.states[f + i].buf, states[i].buf)
+        self._graph_run("F", f, run)
+
+    def _substeps_backward(self, f):
+        """Adjoint substeps f+S-1 .. f; returns the pose gradients of states f+1 .. f+S."""
+        sim, S = self.sim, self.substeps
+        if not self.graphs:
+            pos, rot = [], []
+            for i in range(f + S - 1, f - 1, -1):
+                sim.substep_grad(i)
+                st = sim.states[i + 1]
+                pos.append(st.body_pos_grad.tensor.clone())
+                rot.append(st.body_rot_grad.tensor.clone())
+            return torch.stack(pos[::-1]), torch.stack(rot[::-1])
+        self._pose_buffers(sim.states[f].body_pos.tensor)
+        pg, rg = self._pose_grad_out
+        states = sim.states[f + 1:f + S + 1]
+        dst = list(pg) + list(rg)
+        src = [s.body_pos_grad.tensor for s in states] + [s.body_rot_grad.tensor for s in states]
+
+        def run():
+            for i in range(f + S - 1, f - 1, -1):
+                cur, nxt = sim.states[i], sim.states[i + 1]
+                sim.engine.backward(cur.buf, nxt.buf, cur.grad, nxt.grad, overwrite=True)
+            # the pose gradients of states f+1 .. f+S are complete once the step's adjoint
+            # substeps have run (later adjoint substeps touch state f and earlier only)
+            torch._foreach_copy_(dst, src)
+        self._graph_run("B", f, run)
+        return pg.clone(), rg.clone()
+
     @property
     def diff_forward(self):
         if self._forward_func is None:
@@ -106,28 +198,18 @@ class GradModel:
                 def forward(ctx, s, pos, rot, *past_obs):
                     ctx.s = s
                     ctx.save_for_backward(*[torch.zeros_like(i) for i in past_obs])
-                    sim = model.sim
                     f = s * model.substeps
-                    for i in range(model.substeps):
-                        sim.set_pose(sim.states[f + i + 1], pos[i], rot[i])
-                        sim.substep(f + i, clear_grad=True)
+                    model._substeps_forward(f, pos, rot)
                     return model.get_obs(s + 1, pos.device)
 
                 @staticmethod
                 def backward(ctx, *obs_grad):
                     zero_grads = ctx.saved_tensors
                     s = ctx.s
-                    sim = model.sim
                     model.set_obs_grad(s + 1, *obs_grad)
                     f = s * model.substeps
-                    pos, rot = [], []
-                    for i in range(f + model.substeps - 1, f - 1, -1):
-                        sim.substep_grad(i)
-                        st = sim.states[i + 1]
-                        pos.append(st.body_pos_grad.tensor.clone())
-                        rot.append(st.body_rot_grad.tensor.clone())
-                    pos_grad = torch.stack(pos[::-1]).to(model.device)
-                    rot_grad = torch.stack(rot[::-1]).to(model.device)
+                    pos_grad, rot_grad = model._substeps_backward(f)
+                    pos_grad, rot_grad = pos_grad.to(model.device), rot_grad.to(model.device)
                     return (None, pos_grad, rot_grad) + tuple(zero_grads)
 
             self._forward_func = Forward
