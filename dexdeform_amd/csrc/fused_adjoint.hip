@@ -33,6 +33,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
     const f3 mv{a.y, a.z, a.w};
     const bool act = node_active(m);
     NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
+    const unsigned long long near = near_bodies(P, c.ix, c.iy, c.iz, lane);  // others: no contact here
     const f3 gx = node_position(c.ix, c.iy, c.iz, P.dx);
     const int mode = S.adjoint_mode;
     // forward replay keeping the velocities entering each contact (integrator.cu:678-732)
@@ -45,6 +46,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
       v += P.dt * P.gravity;
       v0 = v;
       for (int b = 0; b < P.nb; ++b) {
+        if (!((near >> b) & 1ull)) continue;
         f3 vin = v;
         if (contact(v, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b), P.dt) && nt < kTapeMax) {
           tape_v[nt] = vin;
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
     for (int b = P.nb - 1; b >= 0; --b) {
       PoseGrad pg = posegrad_zero();
       bool mine = false;
-      if (act) {
+      if (act && ((near >> b) & 1ull)) {
         Body Bd = load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b);
         float infl;
         if (in_contact(prim_sdf(Bd.prim, Bd.args, to_body(Bd.pos, Bd.rot, gx)), Bd.prim[2], infl)) {
@@ -69,7 +71,8 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
           } else {  // tape overflow: replay the contacts before b
             vin = v0;
             for (int b2 = 0; b2 < b; ++b2)
-              contact(vin, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b2), P.dt);
+              if ((near >> b2) & 1ull)
+                contact(vin, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b2), P.dt);
           }
           contact_adj(vin, gx, Bd, P.dt, mode, g, pg);
         }

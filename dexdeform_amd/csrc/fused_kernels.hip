@@ -150,10 +150,9 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
     float4 a = mmv[gi];
     f3 v{0.f, 0.f, 0.f};
-    if (node_active(a.x)) {
-      NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
-      v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P);
-    }
+    const NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
+    const unsigned long long near = near_bodies(P, c.ix, c.iy, c.iz, lane);
+    if (node_active(a.x)) v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P, near);
     gv[gi] = make_float4(v.x, v.y, v.z, 0.f);
     if (SAVE && S.tape_mmv) {
       int slot = 0;
