@@ -182,3 +182,29 @@ def test_chunk_size_for_small_and_large_launches():
         for s, n in ((0, 1000), (2, 300)):
             rows = tab[tab[:, 0] == s]
             assert rows[0, 1] == 0 and rows[-1, 2] == n and np.array_equal(rows[1:, 1], rows[:-1, 2])
+
+
+def test_spatial_order_groups_blocks_then_cells():
+    """spatial_order sorts by 4^3 cell block, then by cell within the block; a random
+    permutation of ordered particles is detected as unordered and sorted back into
+    block/cell order (ties keep their input order)."""
+    from dexdeform_amd.mpm import workloads as W
+    dx = 1 / 64
+    x = W.box_particles(5000, [0.3, 0.1, 0.3], [0.5, 0.3, 0.5], 3, dx)
+    assert W.spatially_ordered(x, dx)
+    xs = x[np.random.default_rng(0).permutation(len(x))]
+    assert not W.spatially_ordered(xs, dx)
+    k = W.spatial_key(xs[W.spatial_order(xs, dx)], dx)
+    assert np.all(k[1:] >= k[:-1])
+    c = np.floor(xs / dx).astype(np.int64)
+    blk = (c >> 2)[W.spatial_order(xs, dx)]
+    # consecutive particles change block only when the (x, y, z) block index increases
+    bk = (blk[:, 0] * 1024 + blk[:, 1]) * 1024 + blk[:, 2]
+    assert np.all(bk[1:] >= bk[:-1])
+
+
+def test_tick_graph_buckets():
+    """Captured server ticks are sized to powers of two (at least the minimum)."""
+    from dexdeform_amd.mpm.serve import _bucket
+    assert _bucket(1, 64) == 64 and _bucket(64, 64) == 64 and _bucket(65, 64) == 128
+    assert _bucket(1000, 64) == 1024 and _bucket(3, 4) == 4 and _bucket(5, 4) == 8
