@@ -17,12 +17,21 @@ from torch.autograd import Function
 
 
 class GradModel:
-    def __init__(self, env, return_grid=(-1,), return_svd=False, graphs=True):
+    def __init__(self, env, return_grid=(-1,), return_svd=False, graphs=True, tape=None):
         """graphs: replay each environment step's substeps (forward, and the adjoint
         substeps of backward) from a HIP graph captured on first use, instead of one
         Python call per substep; the results are the same. Graphs are keyed by the step's
-        state buffers and the simulator's launch tables, and recaptured when those change."""
+        state buffers and the simulator's launch tables, and recaptured when those change.
+
+        tape (with graphs): the forward substeps also keep the SVD temporaries and the
+        grid snapshot of every substep, and the adjoint substeps restore them instead of
+        recomputing P2G and the grid update (120 B per particle plus ~2 KiB per active node
+        block, per substep of the simulator's max_steps). None: on when that memory is at
+        most 16 GiB and a quarter of the device. The adjoint of an environment step reads
+        what its own forward wrote, as it reads the states that forward wrote."""
         self.graphs = bool(graphs)
+        self.tape = tape
+        self._tapes = None
         self._graphs = {}
         self._pose_in = None
         self._pose_grad_out = None
@@ -124,6 +133,25 @@ class GradModel:
             fn()
         self._graphs[key] = g
 
+    def _tape_slots(self):
+        """Per-substep tapes of the simulator's states (allocated on first use), or None."""
+        if not self.graphs or self.tape is False:
+            return None
+        if self._tapes is None:
+            from .rollout import _Tapes
+            eng, sim = self.sim.engine, self.sim
+            caps = eng.tape_capacities()
+            P = eng.n_particles_total
+            slots = sim.max_steps - 1
+            need = slots * (30 * P * 4 + sum(caps) * (2 * 256 * 4 + 4) + 8)
+            if self.tape is None:
+                total = torch.cuda.get_device_properties(sim.device).total_memory
+                self.tape = need <= min(16 << 30, total // 4)
+                if not self.tape:
+                    return None
+            self._tapes = _Tapes(eng, slots, caps)
+        return self._tapes
+
     def _pose_buffers(self, pos):
         S, nb = self.substeps, self.sim.n_bodies
         if self._pose_in is None or self._pose_in[0].device != pos.device:
@@ -154,11 +182,18 @@ class GradModel:
         clear = ([s.grad.body_pos for s in states[:-1]] + [s.grad.body_rot for s in states[:-1]] +
                  list(states[-1].grad.tensors()))
 
+        tapes = self._tape_slots()
+        eng = sim.engine
+
         def run():
             torch._foreach_copy_(dst, src)  # one launch for the environment step's poses
             torch._foreach_zero_(clear)
             for i in range(S):
-                sim.engine.forward(sim.states[f + i].buf, states[i].buf)
+                if tapes is None:
+                    eng.forward(sim.states[f + i].buf, states[i].buf)
+                else:
+                    eng.forward_taped(eng.table_of(sim.states[f + i].buf), eng.table_of(states[i].buf),
+                                      tapes.scene_tabs[f + i])
         self._graph_run("F", f, run)
 
     def _substeps_backward(self, f):
@@ -178,10 +213,17 @@ class GradModel:
         dst = list(pg) + list(rg)
         src = [s.body_pos_grad.tensor for s in states] + [s.body_rot_grad.tensor for s in states]
 
+        tapes = self._tape_slots()
+        eng = sim.engine
+
         def run():
             for i in range(f + S - 1, f - 1, -1):
                 cur, nxt = sim.states[i], sim.states[i + 1]
-                sim.engine.backward(cur.buf, nxt.buf, cur.grad, nxt.grad, overwrite=True)
+                if tapes is None:
+                    eng.backward(cur.buf, nxt.buf, cur.grad, nxt.grad, overwrite=True)
+                else:
+                    eng.backward_taped(eng.table_of(cur.buf), eng.table_of(nxt.buf), eng.table_of(cur.grad),
+                                       eng.table_of(nxt.grad), tapes.scene_tabs[i], overwrite=True)
             # the pose gradients of states f+1 .. f+S are complete once the step's adjoint
             # substeps have run (later adjoint substeps touch state f and earlier only)
             torch._foreach_copy_(dst, src)
