@@ -14,6 +14,14 @@ enum P2GMode { kForward = 0, kRecompute = 1, kForwardTaped = 2, kRecomputeUntape
 //                    (the taped adjoint reads them).
 // kRecomputeUntaped: kRecompute for the scenes whose snapshot was not restored (no
 //                    snapshot, or tape_status[1] set by k_restore); other chunks exit.
+// Taped forward: workgroup 0 of the P2G kernel empties every snapshot of the launch
+// (scenes without particles included, so that no stale snapshot survives) before the
+// grid update, the next kernel, fills them.
+__device__ __forceinline__ void tape_reset(const ddm_batch &B) {
+  for (int si = threadIdx.x; si < live_scenes(B); si += kWG)
+    if (B.scenes[si].tape_mmv) B.scenes[si].tape_status[0] = 0;
+}
+
 // at most 96 VGPRs (a few bytes of spill): five waves per SIMD, as the LDS allows
 template <int MODE>
 __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
@@ -23,6 +31,7 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
   __shared__ Groups gr;
   __shared__ int s_box[6];
   float *const tm = sh.tile[0], *const tx = sh.tile[1], *const ty = sh.tile[2], *const tz = sh.tile[3];
+  if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
   if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
@@ -173,12 +182,6 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
   }
 }
 
-// Taped forward, first: empties every snapshot of the launch (scenes without particles
-// included, so that no stale snapshot survives).
-__global__ __launch_bounds__(kWG) void k_tape_reset(ddm_batch B) {
-  for (int si = threadIdx.x; si < live_scenes(B); si += kWG)
-    if (B.scenes[si].tape_mmv) B.scenes[si].tape_status[0] = 0;
-}
 
 // blocks a scene restores from its snapshot (0 without one, or when it overflowed)
 __device__ __forceinline__ int restorable(const ddm_scene &S) {
@@ -259,6 +262,7 @@ template <int MODE>
 __global__ __launch_bounds__(kWG) void k_det_prep(ddm_batch B, ddm_det D, const ddm_state *__restrict__ cur_tab,
                                                   const ddm_state *__restrict__ next_tab) {
   __shared__ int s_box[6];
+  if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
   if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
   const ddm_scene S = B.scenes[ch.x];
@@ -603,7 +607,6 @@ static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const 
   if (b->n_chunks == 0) return 0;
   const bool det = b->options & DDM_DETERMINISTIC;
   mark(ev, 0, stream);
-  k_tape_reset<<<1, kWG, 0, stream>>>(*b);
   if (det) launch_det_p2g<kForwardTaped>(b, cur, next, stream);
   else k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 1, stream);
