@@ -79,6 +79,7 @@ class Result:
     arrival: float
     admitted: float
     finished: float
+    finite: bool = True  # loss and gradients finite; a failing request does not affect others
 
     @property
     def latency(self) -> float:
@@ -382,6 +383,15 @@ class _Active:
         self.rot_grad.zero_()
         self.x_final = x.clone()
 
+    def check_finite(self):
+        """Enqueue the check that the loss and the gradients are finite; read back through
+        pinned memory once the completion event has fired (no synchronisation here)."""
+        g0 = self.g[self.final_grad_slot]
+        ok = (torch.isfinite(self.loss) & torch.isfinite(self.pos_grad).all() & torch.isfinite(self.rot_grad).all()
+              & torch.isfinite(g0.x).all())
+        self.finite_host = torch.empty((), dtype=torch.bool, pin_memory=True)
+        self.finite_host.copy_(ok, non_blocking=True)
+
     def _addresses(self):
         """Device addresses behind the per-launch state records (fixed for the request)."""
         self.seg_addr = [[int(t.data_ptr()) for t in s.tensors()] for s in self.seg]
@@ -585,6 +595,7 @@ class RolloutServer:
                 continue
             op = a.next_kernel_op()
             if op is None:
+                a.check_finite()
                 a.done_event = torch.cuda.Event()
                 a.done_event.record()
                 finished.append(a)
@@ -615,7 +626,8 @@ class RolloutServer:
                 if a.inv is not None:  # back to the request's particle order
                     x_final, x0_grad = x_final[a.inv], x0_grad[a.inv]
                 self.results[a.req.rid] = Result(a.req.rid, float(a.loss_host.item()), x_final, a.pos_grad,
-                                                 a.rot_grad, x0_grad, a.req.arrival, a.admitted, fin)
+                                                 a.rot_grad, x0_grad, a.req.arrival, a.admitted, fin,
+                                                 bool(a.finite_host.item()))
                 self.used_bytes -= request_bytes(a.req)
                 self.arena.release(a.blk_begin, a.n_blocks)
             else:

@@ -176,6 +176,27 @@ def test_server_sorts_particles_given_in_random_order():
     assert el <= 1e-5 and ex <= 1e-5 and e0 <= 1e-3 and eg <= 1e-3
 
 
+def test_failing_request_is_flagged_and_isolated():
+    """A request whose loss is not finite (a NaN in its target, injected from the host:
+    positions stay finite, only its loss and gradients are NaN) is reported with
+    finite=False; the requests served with it get their stand-alone results."""
+    reqs = _requests()[:3]
+    bad = reqs[1]
+    bad.target = bad.target.copy()
+    bad.target[7, 1] = np.nan
+    srv = RolloutServer(device=_dev(), arena_blocks=16384, memory_budget=8 << 30)
+    res = srv.run(reqs)
+    assert not res[bad.rid].finite and not np.isfinite(res[bad.rid].loss)
+    for r in (reqs[0], reqs[2]):
+        alone = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30).run([_requests()[r.rid]])[r.rid]
+        b = res[r.rid]
+        assert b.finite and alone.finite
+        el = abs(alone.loss - b.loss) / abs(alone.loss)
+        eg = rel_err(b.pos_grad.cpu().numpy(), alone.pos_grad.cpu().numpy())
+        print(f"request {r.rid} next to a failing request: loss {el:.1e} pose grad {eg:.1e}")
+        assert el <= 1e-4 and eg <= 1e-3
+
+
 def test_server_matches_direct_rollout():
     r = _requests()[1]
     srv = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30)
