@@ -238,10 +238,11 @@ def tape_capacity(req: Request) -> int:
     return min((gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4), req.n // 32 + 64)
 
 
-def request_bytes(req: Request) -> int:
-    """Device bytes a request holds while active."""
+def request_bytes(req: Request, K: Optional[int] = None) -> int:
+    """Device bytes a request holds while active, with checkpoints every K substeps
+    (default: the request's interval; K = T keeps every state and tapes every substep)."""
     T = req.T
-    K = req.checkpoint_every or T
+    K = K or req.checkpoint_every or T
     n_states = (T // K + 1 if K != T else 0) + (K + 1) + 2
     per_state = req.n * 24 * 4
     temps = req.n * 5 * 4 + K * (req.n * 30 * 4 + tape_capacity(req) * (2 * 1024 + 4) + 8)
@@ -255,7 +256,7 @@ def request_bytes(req: Request) -> int:
 class _Active:
     """Device-side state and operation schedule of an admitted request."""
 
-    def __init__(self, server: "RolloutServer", req: Request, blk_begin: int, now: float):
+    def __init__(self, server: "RolloutServer", req: Request, blk_begin: int, now: float, K: int):
         dev = server.device
         self.req = req
         self.admitted = now
@@ -264,9 +265,10 @@ class _Active:
         gd = [int(g) for g in sp.grid_dim]
         self.n_blocks = (gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4)
         n, nb, T = req.n, sp.n_bodies, req.T
-        self.K = req.checkpoint_every or T
+        self.K = int(K)
         if T % self.K:
             raise ValueError("checkpoint_every must divide the horizon")
+        self.bytes = request_bytes(req, self.K)
         f = lambda a: _to_device(a, dev)
         # particles not in spatial order (4^3 block, then cell) are sorted for the kernels;
         # results are returned in the request's order
@@ -441,7 +443,12 @@ class _Active:
 
 class RolloutServer:
     def __init__(self, device="cuda:0", arena_blocks: int = 64 * 13824, memory_budget: int = 64 << 30,
-                 max_active: int = 256, use_graphs: bool = True):
+                 max_active: int = 256, use_graphs: bool = True, full_tape_bytes: Optional[int] = None):
+        """full_tape_bytes: a request whose whole trajectory (every state, every substep
+        taped) takes at most this many bytes is run without checkpoints -- 2T substep
+        operations instead of 3T, same results up to FP32 rounding; larger ones use their
+        checkpoint interval. Default: an eighth of the memory budget; 0 always honours
+        the request's interval."""
         self.lib = _lib.load()
         self.device = torch.device(device)
         self.arena = BlockArena(arena_blocks)
@@ -452,6 +459,7 @@ class RolloutServer:
         self.block_list = torch.zeros(arena_blocks, 2, dtype=torch.int32, device=self.device)
         self.block_count = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.memory_budget = memory_budget
+        self.full_tape_bytes = memory_budget // 8 if full_tape_bytes is None else int(full_tape_bytes)
         self.max_active = max_active
         self.waiting: List[Request] = []
         self.active: List[_Active] = []
@@ -478,6 +486,12 @@ class RolloutServer:
         req.spec.validate()
         self.waiting.append(req)
 
+    def interval(self, req: Request) -> int:
+        """Checkpoint interval the request runs with (see full_tape_bytes)."""
+        if request_bytes(req, req.T) <= self.full_tape_bytes:
+            return req.T
+        return req.checkpoint_every or req.T
+
     def _now(self):
         return time.perf_counter() - self.t0
 
@@ -486,7 +500,8 @@ class RolloutServer:
             req = self.waiting[0]
             if req.arrival > now:
                 break
-            need = request_bytes(req)
+            K = self.interval(req)
+            need = request_bytes(req, K)
             if self.used_bytes + need > self.memory_budget:
                 break
             gd = req.spec.grid_dim
@@ -497,7 +512,7 @@ class RolloutServer:
             self.waiting.pop(0)
             self.used_bytes += need
             t0 = time.perf_counter()
-            self.active.append(_Active(self, req, b0, now))
+            self.active.append(_Active(self, req, b0, now, K))
             self.host_s["admit"] += time.perf_counter() - t0
 
     # ------------------------------------------------------------------ launches
@@ -628,7 +643,7 @@ class RolloutServer:
                 self.results[a.req.rid] = Result(a.req.rid, float(a.loss_host.item()), x_final, a.pos_grad,
                                                  a.rot_grad, x0_grad, a.req.arrival, a.admitted, fin,
                                                  bool(a.finite_host.item()))
-                self.used_bytes -= request_bytes(a.req)
+                self.used_bytes -= a.bytes
                 self.arena.release(a.blk_begin, a.n_blocks)
             else:
                 still.append(a)

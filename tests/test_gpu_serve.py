@@ -96,24 +96,28 @@ def _requests():
     return reqs
 
 
-def test_server_results_alone_equal_mixed():
+@pytest.mark.parametrize("full_tape_bytes", [0, None])
+def test_server_results_alone_equal_mixed(full_tape_bytes):
+    """0: every request runs with its checkpoint interval (recomputed segments); None:
+    these small requests keep their whole trajectory (no recompute)."""
     reqs = _requests()
     alone = {}
     for r in reqs:
-        srv = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30)
+        srv = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30, full_tape_bytes=full_tape_bytes)
         alone[r.rid] = srv.run([r])[r.rid]
     # mixed: staggered arrivals so requests join while others are mid-rollout
     mixed_reqs = _requests()
     for k, r in enumerate(mixed_reqs):
         r.arrival = 0.002 * k
     # the arena holds every request's grid at once (1728 + 4096 + 512 + 1728 + 4096 blocks)
-    srv = RolloutServer(device=_dev(), arena_blocks=16384, memory_budget=8 << 30)
+    srv = RolloutServer(device=_dev(), arena_blocks=16384, memory_budget=8 << 30, full_tape_bytes=full_tape_bytes)
+    assert all((srv.interval(r) == r.T) == (full_tape_bytes is None) for r in reqs if r.checkpoint_every)
     mixed = srv.run(mixed_reqs)
     stats = latency_stats(mixed)
     print("mixed latency", stats, "ticks", srv.ticks)
     # requests overlap on the device: several are admitted and unfinished at once, and the
     # server needs fewer ticks than one-at-a-time execution (the sum of their operations)
-    ops = [2 * r.T + (r.T if r.checkpoint_every else 0) for r in reqs]
+    ops = [2 * r.T + (r.T if srv.interval(r) < r.T else 0) for r in reqs]
     events = sorted([(r.admitted, 1) for r in mixed.values()] + [(r.finished, -1) for r in mixed.values()])
     live, most = 0, 0
     for _, d in events:
