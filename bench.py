@@ -100,18 +100,22 @@ def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False, d
 def serve_benchmark(dev, rates, seconds, seed, graphs=True):
     """Served rollouts: for each Poisson arrival rate, a stream of uneven requests
     (particles U[5k,100k], grid 48^3/64^3/96^3, horizon U[20,200] with checkpoints every
-    10 substeps, forward + backward each) through the continuous-batching server.
+    10 substeps, forward + backward each) through the continuous-batching server; then
+    200 such requests arriving at once ("burst": throughput with the server saturated).
     Latency = completion - arrival; throughput = particle-substeps (forward + adjoint
     counted once) per second of wall time from the first arrival to the last completion."""
     import torch
     from dexdeform_amd.mpm import workloads as W
     from dexdeform_amd.mpm.serve import Request, RolloutServer, latency_stats
     out = []
-    for k, rate in enumerate(rates):
-        n_requests = max(24, int(round(rate * seconds)))
-        stream = W.served_stream(rate, n_requests, seed=seed + 1000 * k)
+    # the rates, then a burst of 200 requests arriving at once (the server saturated)
+    for k, rate in enumerate(list(rates) + ["burst"]):
+        burst = rate == "burst"
+        n_requests = 200 if burst else max(24, int(round(rate * seconds)))
+        stream = W.served_stream(200.0 if burst else rate, n_requests, seed=seed + 1000 * k)
         reqs = [Request(i, d["scene"].spec, d["scene"].x0, d["pose_pos"], d["pose_rot"], d["target"],
-                        arrival=d["arrival"], checkpoint_every=d["checkpoint_every"]) for i, d in enumerate(stream)]
+                        arrival=0.0 if burst else d["arrival"], checkpoint_every=d["checkpoint_every"])
+                for i, d in enumerate(stream)]
         srv = RolloutServer(device=dev, use_graphs=graphs)
         t0 = time.perf_counter()
         res = srv.run(reqs)
@@ -122,7 +126,7 @@ def serve_benchmark(dev, rates, seconds, seed, graphs=True):
                     "throughput": ps / wall, "wall_s": wall, "ticks": srv.ticks, "graph_replays": srv.replays,
                     "graphs_captured": len(srv.graphs),
                     "host_s": {k: round(v, 4) for k, v in srv.host_s.items()}})
-        log(f"serve rate {rate}/s: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
+        log(f"serve rate {rate}{'' if burst else '/s'}: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
             f"throughput {ps / wall:.3e} particle-substeps/s")
         del srv, res
         torch.cuda.empty_cache()

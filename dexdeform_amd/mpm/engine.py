@@ -118,12 +118,14 @@ def chunk_table(counts: Sequence[int], scene_ids: Optional[Sequence[int]] = None
     <= chunk pieces (chunk <= CHUNK: one workgroup of CHUNK threads per row)."""
     if not 0 < chunk <= CHUNK:
         raise ValueError(f"chunk must be in (0, {CHUNK}]")
-    rows = []
     ids = range(len(counts)) if scene_ids is None else scene_ids
+    parts = []
     for s, n in zip(ids, counts):
-        for c in range(0, int(n), chunk):
-            rows.append((s, c, min(c + chunk, int(n)), 0))
-    return np.asarray(rows, dtype=np.int32).reshape(-1, 4)
+        first = np.arange(0, int(n), chunk, dtype=np.int32)
+        t = np.zeros((len(first), 4), np.int32)
+        t[:, 0], t[:, 1], t[:, 2] = s, first, np.minimum(first + chunk, int(n))
+        parts.append(t)
+    return np.concatenate(parts) if parts else np.zeros((0, 4), np.int32)
 
 
 def state_table(records, device) -> torch.Tensor:

@@ -45,7 +45,7 @@ import torch
 
 from .. import _lib
 from .engine import CHUNK, SceneSpec, chunk_size_for, chunk_table
-from .workloads import spatial_order, spatially_ordered
+from .workloads import needs_sort, spatial_order
 
 
 @dataclass
@@ -139,11 +139,43 @@ class _View:
         self.flat.zero_()  # one fill kernel for the four arrays
 
 
-def _state_sets(n, count, device):
+class _StateSets:
     """count particle-state sets in one uninitialised allocation (every set is written
-    before it is read: by a substep, a checkpoint copy or an explicit initialisation)."""
-    block = torch.empty(count, 24 * n, dtype=torch.float32, device=device)
-    return [_View(block[i], n) for i in range(count)]
+    before it is read: by a substep, a checkpoint copy or an explicit initialisation).
+    Views are made on first access; addresses() gives the x, v, F, C addresses of every
+    set without making them."""
+
+    def __init__(self, n, count, device, start=0, block=None):
+        self.n, self.start, self.count = n, start, count
+        self.block = torch.empty(count, 24 * n, dtype=torch.float32, device=device) if block is None else block
+        self._views = {}
+
+    def __len__(self):
+        return self.count
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            a, b, step = i.indices(self.count)
+            assert step == 1
+            return _StateSets(self.n, b - a, None, self.start + a, self.block)
+        if i < 0:
+            i += self.count
+        if not 0 <= i < self.count:
+            raise IndexError(i)
+        v = self._views.get(i)
+        if v is None:
+            v = self._views[i] = _View(self.block[self.start + i], self.n)
+        return v
+
+    def addresses(self) -> np.ndarray:
+        """[count, 4] int64: x, v, F, C addresses of each set."""
+        base = int(self.block.data_ptr()) + self.start * 24 * self.n * 4
+        rows = base + np.arange(self.count, dtype=np.int64)[:, None] * (24 * self.n * 4)
+        return rows + np.array([0, 3, 6, 15], dtype=np.int64)[None] * (self.n * 4)
+
+
+def _state_sets(n, count, device):
+    return _StateSets(n, count, device)
 
 
 def _to_device(a, device, dtype=np.float32) -> torch.Tensor:
@@ -151,6 +183,19 @@ def _to_device(a, device, dtype=np.float32) -> torch.Tensor:
     allocator keeps it until the copy has run)."""
     h = torch.from_numpy(np.ascontiguousarray(a, dtype)).pin_memory()
     return h.to(device, non_blocking=True)
+
+
+def _upload_packed(arrays, device) -> List[torch.Tensor]:
+    """Upload float32 arrays through one pinned host buffer and one asynchronous copy;
+    returns device views of the arrays' shapes, each starting on a 256-byte boundary."""
+    arrays = [np.asarray(a, np.float32) for a in arrays]
+    offs = np.cumsum([0] + [(a.size + 63) // 64 * 64 for a in arrays])
+    host = torch.empty(int(offs[-1]), dtype=torch.float32, pin_memory=True)
+    hn = host.numpy()
+    for a, o in zip(arrays, offs):
+        hn[o:o + a.size] = a.reshape(-1)
+    d = host.to(device, non_blocking=True)
+    return [d[o:o + a.size].view(a.shape) for a, o in zip(arrays, offs)]
 
 
 class _TickTables:
@@ -269,16 +314,19 @@ class _Active:
         if T % self.K:
             raise ValueError("checkpoint_every must divide the horizon")
         self.bytes = request_bytes(req, self.K)
-        f = lambda a: _to_device(a, dev)
-        # particles not in spatial order (4^3 block, then cell) are sorted for the kernels;
+        # particles far from spatial order (4^3 block, then cell) are sorted for the kernels;
         # results are returned in the request's order
-        self.order = None if spatially_ordered(req.x0, sp.dx) else spatial_order(req.x0, sp.dx)
+        self.order = spatial_order(req.x0, sp.dx) if needs_sort(req.x0, sp.dx) else None
         srt = (lambda a: np.asarray(a)[self.order]) if self.order is not None else (lambda a: a)
-        self.inv = (torch.as_tensor(np.argsort(self.order), device=dev) if self.order is not None else None)
-        self.mass, self.vol, self.mly = (f(srt(sp.mass)), f(srt(sp.vol)),
-                                         f(srt(np.asarray(sp.mu_lam_yield).reshape(-1, 3))))
-        self.prim = f(np.asarray(sp.body_prim).reshape(-1, 4)) if nb else None
-        self.args = f(np.asarray(sp.body_args).reshape(-1, 4)) if nb else None
+        self.inv = _to_device(np.argsort(self.order), dev, np.int64) if self.order is not None else None
+        up = [srt(sp.mass), srt(sp.vol), srt(np.asarray(sp.mu_lam_yield).reshape(-1, 3)),
+              srt(np.asarray(req.x0).reshape(-1, 3)), srt(np.asarray(req.target).reshape(-1, 3))]
+        if nb:
+            up += [np.asarray(sp.body_prim).reshape(-1, 4), np.asarray(sp.body_args).reshape(-1, 4),
+                   np.asarray(req.pose_pos).reshape(T + 1, nb, 3), np.asarray(req.pose_rot).reshape(T + 1, nb, 4)]
+        dv = _upload_packed(up, dev)
+        self.mass, self.vol, self.mly, x0, self.target = dv[:5]
+        self.prim, self.args = (dv[5], dv[6]) if nb else (None, None)
         # taped adjoint: per substep of a segment, the SVD temporaries and a grid snapshot
         # (written by the segment's taped forward pass, read by its adjoint substeps)
         self.tape_cap = tape_capacity(req)
@@ -287,13 +335,12 @@ class _Active:
         self.tape_v = torch.empty_like(self.tape_mmv)
         self.tape_list = torch.empty(self.K, self.tape_cap, dtype=torch.int32, device=dev)
         self.tape_status = torch.zeros(self.K, 2, dtype=torch.int32, device=dev)
-        nb_rows = max(nb, 1)
-        self.pose_pos = torch.zeros(T + 1, nb_rows, 3, device=dev)
-        self.pose_rot = torch.zeros(T + 1, nb_rows, 4, device=dev)
-        self.pose_rot[..., 0] = 1
         if nb:
-            self.pose_pos.copy_(f(req.pose_pos))
-            self.pose_rot.copy_(f(req.pose_rot))
+            self.pose_pos, self.pose_rot = dv[7], dv[8]
+        else:
+            self.pose_pos = torch.zeros(T + 1, 1, 3, device=dev)
+            self.pose_rot = torch.zeros(T + 1, 1, 4, device=dev)
+            self.pose_rot[..., 0] = 1
         self.pos_grad = torch.zeros_like(self.pose_pos)
         self.rot_grad = torch.zeros_like(self.pose_rot)
         self.full = self.K == T
@@ -303,14 +350,13 @@ class _Active:
         self.seg = sets[n_ckpt:n_ckpt + self.K + 1]
         self.g = sets[n_ckpt + self.K + 1:]
         s0 = self.seg[0]
-        s0.x.copy_(f(srt(req.x0)))
+        s0.x.copy_(x0)
         s0.v.zero_()
         s0.C.zero_()
         s0.F.zero_()
         s0.F.diagonal(dim1=1, dim2=2).fill_(1.0)
         if not self.full:
             self.ckpt[0].copy_(self.seg[0])
-        self.target = f(srt(req.target))
         self.record = np.zeros(1, dtype=_lib.SCENE_DTYPE)[0]
         r = self.record
         r["p_count"] = n
@@ -395,21 +441,25 @@ class _Active:
         self.finite_host.copy_(ok, non_blocking=True)
 
     def _addresses(self):
-        """Device addresses behind the per-launch state records (fixed for the request)."""
-        self.seg_addr = [[int(t.data_ptr()) for t in s.tensors()] for s in self.seg]
-        self.g_addr = [[int(t.data_ptr()) for t in g.tensors()] for g in self.g]
+        """Device addresses behind the per-launch state records and the per-slot taped scene
+        records (fixed for the request), computed from the allocations' base addresses."""
+        self.seg_addr = self.seg.addresses().tolist()
+        self.g_addr = self.g.addresses().tolist()
         rows = self.pose_pos.shape[1]
         self.pose_addr = (int(self.pose_pos.data_ptr()), int(self.pose_rot.data_ptr()), rows * 12, rows * 16)
         self.pgrad_addr = (int(self.pos_grad.data_ptr()), int(self.rot_grad.data_ptr()))
         self.rec_bytes = np.array([self.record], dtype=_lib.SCENE_DTYPE).view(np.uint8)
-        self.rec_taped = []
-        for j in range(self.K):
-            r = self.record.copy()
-            self._temps_of(r, j)
-            r["tape_mmv"], r["tape_v"] = int(self.tape_mmv[j].data_ptr()), int(self.tape_v[j].data_ptr())
-            r["tape_list"], r["tape_status"] = int(self.tape_list[j].data_ptr()), int(self.tape_status[j].data_ptr())
-            r["tape_capacity"] = self.tape_cap
-            self.rec_taped.append(np.array([r], dtype=_lib.SCENE_DTYPE).view(np.uint8))
+        K, n, f4 = self.K, self.req.n, 4
+        j = np.arange(K, dtype=np.uint64)
+        recs = np.repeat(np.array([self.record], dtype=_lib.SCENE_DTYPE), K)
+        slot = lambda t: np.uint64(t.data_ptr()) + j * np.uint64(t.stride(0) * t.element_size())
+        temps = slot(self.tape_temps)
+        recs["tmp_Ft"], recs["tmp_U"] = temps, temps + np.uint64(9 * n * f4)
+        recs["tmp_V"], recs["tmp_sig"] = temps + np.uint64(18 * n * f4), temps + np.uint64(27 * n * f4)
+        recs["tape_mmv"], recs["tape_v"] = slot(self.tape_mmv), slot(self.tape_v)
+        recs["tape_list"], recs["tape_status"] = slot(self.tape_list), slot(self.tape_status)
+        recs["tape_capacity"] = self.tape_cap
+        self.rec_taped = recs.view(np.uint8).reshape(K, -1)
 
     def record_for(self, op):
         """Scene record of a kernel operation: slot j's taped record, or the plain one."""
@@ -442,13 +492,15 @@ class _Active:
 
 
 class RolloutServer:
-    def __init__(self, device="cuda:0", arena_blocks: int = 64 * 13824, memory_budget: int = 64 << 30,
+    def __init__(self, device="cuda:0", arena_blocks: int = 64 * 13824, memory_budget: Optional[int] = None,
                  max_active: int = 256, use_graphs: bool = True, full_tape_bytes: Optional[int] = None):
-        """full_tape_bytes: a request whose whole trajectory (every state, every substep
-        taped) takes at most this many bytes is run without checkpoints -- 2T substep
-        operations instead of 3T, same results up to FP32 rounding; larger ones use their
-        checkpoint interval. Default: an eighth of the memory budget; 0 always honours
-        the request's interval."""
+        """memory_budget: device bytes the admitted requests may hold (default: 60 % of the
+        device's memory). full_tape_bytes: a request whose whole trajectory (every state,
+        every substep taped) takes at most this many bytes, admitted while at most half
+        the budget is in use, runs without checkpoints -- 2T substep operations instead of
+        3T, same results up to FP32 rounding; otherwise it uses its checkpoint interval
+        (more requests fit). Default: an eighth of the budget; 0 always honours the
+        request's interval."""
         self.lib = _lib.load()
         self.device = torch.device(device)
         self.arena = BlockArena(arena_blocks)
@@ -458,8 +510,10 @@ class RolloutServer:
         self.block_flags = torch.zeros(arena_blocks, dtype=torch.int32, device=self.device)
         self.block_list = torch.zeros(arena_blocks, 2, dtype=torch.int32, device=self.device)
         self.block_count = torch.zeros(1, dtype=torch.int32, device=self.device)
-        self.memory_budget = memory_budget
-        self.full_tape_bytes = memory_budget // 8 if full_tape_bytes is None else int(full_tape_bytes)
+        if memory_budget is None:
+            memory_budget = int(0.6 * torch.cuda.get_device_properties(self.device).total_memory)
+        self.memory_budget = int(memory_budget)
+        self.full_tape_bytes = self.memory_budget // 8 if full_tape_bytes is None else int(full_tape_bytes)
         self.max_active = max_active
         self.waiting: List[Request] = []
         self.active: List[_Active] = []
@@ -487,8 +541,10 @@ class RolloutServer:
         self.waiting.append(req)
 
     def interval(self, req: Request) -> int:
-        """Checkpoint interval the request runs with (see full_tape_bytes)."""
-        if request_bytes(req, req.T) <= self.full_tape_bytes:
+        """Checkpoint interval the request would run with if admitted now (see
+        full_tape_bytes)."""
+        full = request_bytes(req, req.T)
+        if full <= self.full_tape_bytes and self.used_bytes + full <= self.memory_budget // 2:
             return req.T
         return req.checkpoint_every or req.T
 
@@ -496,7 +552,12 @@ class RolloutServer:
         return time.perf_counter() - self.t0
 
     def _admit(self, now):
+        # admission is host work (allocations, uploads, tables); spend at most ~2 ms of it
+        # per tick while requests are running, so the device keeps receiving ticks
+        t_start = time.perf_counter()
         while self.waiting and len(self.active) < self.max_active:
+            if self.active and time.perf_counter() - t_start > 2e-3:
+                break
             req = self.waiting[0]
             if req.arrival > now:
                 break

@@ -261,10 +261,9 @@ class MPMSimulator:
     def _choose_order(self, x):
         """Keep the caller's order unless it is far from spatial (more than 10 % of
         neighbouring particles out of block/cell order, e.g. randomly sampled)."""
-        from .workloads import spatial_key, spatial_order
+        from .workloads import needs_sort, spatial_order
         x = np.asarray(x, np.float32)
-        k = spatial_key(x, self.dx)
-        if not self.sort_particles or len(k) < 2 or np.mean(k[1:] < k[:-1]) <= 0.1:
+        if not self.sort_particles or not needs_sort(x, self.dx):
             self._perm = self._inv = self._perm_np = self._inv_np = None
         else:
             self._perm_np = spatial_order(x, self.dx)
