@@ -48,10 +48,10 @@ KERNEL_BYTES = {
 # kernel changes; reported as roofline.traffic for the default workload only.
 TRAFFIC_BYTES = {
     "p2g_fwd": 275, "grid_fwd": 10, "g2p_fwd": 88, "p2g_recompute": 316, "grid_recompute": 7, "g2p_adj": 163,
-    "grid_adj": 17, "p2g_adj": 462,
+    "grid_adj": 18, "p2g_adj": 462,
     # taped rollout
-    "taped_p2g_fwd": 377, "taped_grid_fwd": 17, "taped_g2p_fwd": 88, "taped_restore": 13, "taped_g2p_adj": 163,
-    "taped_grid_adj": 17, "taped_p2g_adj": 462,
+    "taped_p2g_fwd": 465, "taped_grid_fwd": 17, "taped_g2p_fwd": 88, "taped_restore": 13, "taped_g2p_adj": 163,
+    "taped_grid_adj": 18, "taped_p2g_adj": 462,
 }
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
