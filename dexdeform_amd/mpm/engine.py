@@ -13,7 +13,9 @@ sequence (mpm/simulator.py:561-571); the backward replaces
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
+import gc
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -100,6 +102,22 @@ class StateBuffers:
         for a, b in zip(self.tensors(), other.tensors()):
             a.copy_(b)
         return self
+
+
+@contextlib.contextmanager
+def gc_paused():
+    """Python's automatic cyclic garbage collection paused (for a HIP graph capture): a
+    collection during a capture could free a dead graph held in a reference cycle, and a
+    graph destroyed while a stream is capturing fails (hipErrorStreamCaptureUnsupported,
+    which the graph's destructor turns into std::terminate). Dead cycles are collected
+    after the capture, as usual."""
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
 
 
 def chunk_size_for(n_particles: int) -> int:

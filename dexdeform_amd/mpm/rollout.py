@@ -27,7 +27,7 @@ from typing import List, Optional
 import torch
 
 from .. import _lib
-from .engine import Engine
+from .engine import Engine, gc_paused
 
 
 class _Particles:
@@ -224,7 +224,7 @@ class Rollout:
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with gc_paused(), torch.cuda.graph(g):
             self.forward()
             self.backward(x_grad=x_grad)
         self._graph = g

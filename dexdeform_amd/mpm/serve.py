@@ -44,7 +44,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .engine import CHUNK, SceneSpec, chunk_size_for, chunk_table
+from .engine import CHUNK, SceneSpec, chunk_size_for, chunk_table, gc_paused
 from .workloads import needs_sort, spatial_order
 
 
@@ -650,7 +650,7 @@ class RolloutServer:
             # first tick of this shape: capture its launches on a side stream (capture runs
             # nothing, and the launches allocate nothing, so queued work need not drain)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.stream(self.capture_stream):
+            with gc_paused(), torch.cuda.stream(self.capture_stream):
                 g.capture_begin(capture_error_mode="thread_local")
                 try:
                     self._launch_tick(key, live=True)

@@ -15,6 +15,8 @@ from __future__ import annotations
 import torch
 from torch.autograd import Function
 
+from .engine import gc_paused
+
 
 class GradModel:
     def __init__(self, env, return_grid=(-1,), return_svd=False, graphs=True, tape=None):
@@ -129,7 +131,7 @@ class GradModel:
             return
         fn()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with gc_paused(), torch.cuda.graph(g):
             fn()
         self._graphs[key] = g
 

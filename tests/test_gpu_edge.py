@@ -9,7 +9,9 @@
   positions lie in [0.5, 1), where the translation by 0.25 is exact in FP32, so only
   the atomic order differs: 1e-4 / 1e-3);
 * the DDM_EINVAL path: a batch asking for the deterministic mode without its workspace
-  is rejected before any launch.
+  is rejected before any launch;
+* a HIP graph capture while a dead graph sits in a reference cycle: the capture pauses
+  automatic garbage collection, and its replay matches the eager pass.
 """
 import ctypes
 
@@ -103,3 +105,63 @@ def test_deterministic_option_without_workspace_is_rejected():
     st = eng.lib.ddm_substep_forward(ctypes.byref(b), eng.table_of(cur).data_ptr(), eng.table_of(nxt).data_ptr(),
                                      _lib.stream_ptr())
     assert st == _lib.DDM_EINVAL
+
+
+def test_capture_with_a_dead_graph_in_a_reference_cycle():
+    """A dead reference cycle that holds a captured HIP graph is not collected during a
+    later capture (the graph's destructor fails while a stream is capturing and
+    terminates the process): captures pause Python's automatic garbage collection. The
+    cycle is dropped inside the capture with the collection threshold at 1, so an
+    unpaused collector would run there."""
+    import gc
+    dev = _dev()
+    sc = _scene(2000, 32, 1 / 32, [0.4, 0.1, 0.4], [0.6, 0.2, 0.6], [0.5, 0.25, 0.5], seed=6)
+    T = 4
+
+    def rollout():
+        eng = Engine([sc.spec], device=dev)
+        ro = Rollout(eng, T, tape=True)
+        ro.set_initial(torch.tensor(sc.x0, device=dev))
+        t = np.arange(T + 1, dtype=np.float32)[:, None, None]
+        pos = (sc.body_pos[None] + t * sc.spec.dt * np.float32([0.3, -0.2, 0.1])).astype(np.float32)
+        rot = np.repeat(sc.body_rot[None], T + 1, 0)
+        ro.set_poses(torch.tensor(pos, device=dev), torch.tensor(rot, device=dev))
+        return eng, ro
+
+    seed = torch.full((sc.n, 3), 1e-3, device=dev)
+    e1, r1 = rollout()
+    r1.capture(seed)
+    cycle = [r1._graph]
+    cycle.append(cycle)
+    r1._graph = None
+    del e1, r1
+    e2, r2 = rollout()
+    fwd = r2.forward
+    dropped = []
+
+    def forward_dropping_the_cycle():
+        nonlocal cycle
+        if cycle is not None and torch.cuda.is_current_stream_capturing():
+            cycle = None  # the graph is now garbage in a cycle
+            dropped.append([[] for _ in range(100)])  # allocations that trigger collections
+        return fwd()
+
+    r2.forward = forward_dropping_the_cycle
+    thresholds = gc.get_threshold()
+    gc.set_threshold(1, 1, 1)
+    try:
+        r2.capture(seed)
+    finally:
+        gc.set_threshold(*thresholds)
+    assert dropped, "the cycle was not dropped inside the capture"
+    gc.collect()  # outside the capture: the dead graph goes now
+    r2.replay()
+    x_graph = r2.final_state().x.clone()
+    g_graph = r2.initial_grad().x.clone()
+    x_eager = fwd().x.clone()
+    r2.backward(x_grad=seed)
+    torch.cuda.synchronize()
+    ex, eg = rel_err(x_graph.cpu().numpy(), x_eager.cpu().numpy()), rel_err(g_graph.cpu().numpy(),
+                                                                              r2.initial_grad().x.cpu().numpy())
+    print(f"replay after a capture with a dead graph cycle vs eager: x {ex:.1e} x0 grad {eg:.1e}")
+    assert ex <= 1e-5 and eg <= 1e-4
