@@ -201,6 +201,15 @@ def test_spatial_order_groups_blocks_then_cells():
     # consecutive particles change block only when the (x, y, z) block index increases
     bk = (blk[:, 0] * 1024 + blk[:, 1]) * 1024 + blk[:, 2]
     assert np.all(bk[1:] >= bk[:-1])
+    # needs_sort: random order is flagged; sorted input, and sorted input with a few
+    # neighbours swapped (under 10 % of adjacent pairs out of order), is not
+    assert W.needs_sort(xs, dx)
+    assert not W.needs_sort(x, dx)
+    xn = x.copy()
+    i = np.arange(0, len(x) - 1, 40)
+    xn[i], xn[i + 1] = x[i + 1], x[i]
+    assert not W.needs_sort(xn, dx)
+    assert not W.needs_sort(x[:1], dx)
 
 
 def test_tick_graph_buckets():
