@@ -13,6 +13,11 @@ itself (eps = 0.1 in action units, i.e. joint-angle changes below 7e-4 rad), per
 coordinate and along the gradient direction for a sweep of step sizes. Their rounding noise (FP32 state, atomic
 order) is reported beside the error: the same kernels are checked against FP64 finite differences
 at the smaller hand-contact size by tests/test_gpu_hand.py.
+
+At reduced particle counts the adjoint itself is compared with the FP64 oracle's:
+
+    python tests/check_large_scene.py --particles 100000 --save-grad g.npy   (GPU)
+    python tests/check_large_scene.py --particles 100000 --oracle g.npy      (CPU, minutes)
 """
 import argparse
 import json
@@ -43,7 +48,13 @@ def main():
     ap.add_argument("--no-tape", action="store_true", help="recompute adjoint without a segment tape")
     ap.add_argument("--dir-eps", default="0.1,0.03,0.01,0.003,0.001",
                     help="step sizes of the directional finite differences along the gradient")
+    ap.add_argument("--save-grad", default=None, help="write the action gradient (.npy) for --oracle")
+    ap.add_argument("--oracle", default=None, metavar="GRAD_NPY",
+                    help="CPU only: compare a gradient written by --save-grad (same --particles and "
+                         "--env-steps) with the FP64 oracle's adjoint of the same rollout")
     a = ap.parse_args()
+    if a.oracle:
+        return oracle_compare(a)
     dev = torch.device("cuda:0")
     sc, tree, q0 = W.large_scene(a.particles)
     sc.spec.adjoint_mode = 1
@@ -74,6 +85,8 @@ def main():
     loss, g = loss_of(acts, grad=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    if a.save_grad:
+        np.save(a.save_grad, g)
     rng = np.random.default_rng(5)
     coords = [(int(rng.integers(a.env_steps)), int(rng.integers(22))) for _ in range(a.coords)]
     fd, noise = [], []
@@ -104,6 +117,33 @@ def main():
            "directional_rel_err": {str(h): float(abs(v - gn) / gn) for h, v in sweep.items()},
            "loss_run_to_run_spread": float(np.ptp(l0)), "fd_noise_estimate": float(np.ptp(l0) / (2 * a.eps))}
     print(json.dumps(out), flush=True)
+
+
+def oracle_compare(a):
+    """FP64 oracle adjoint (the same scene, actions, target and exact adjoint mode, all
+    states kept in memory) against the GPU's action gradient; prints one JSON line."""
+    from oracle import oracle as O
+    from tests.helpers import initial_state, oracle_scene
+    sc, tree, q0 = W.large_scene(a.particles)
+    sc.spec.adjoint_mode = 1
+    acts = W.hand_actions(a.env_steps)
+    target = W.plain_target(sc.x0)
+    g_gpu = np.load(a.oracle)
+    A = torch.tensor(acts, dtype=torch.float64, requires_grad=True)
+    P, R = tree.to(dtype=torch.float64).trajectory(q0.double(), A, SUB)
+    osim = O.OracleSim(oracle_scene(sc), np.float64, adjoint="exact", omp=True)
+    t0 = time.perf_counter()
+    states = osim.rollout(initial_state(sc).astype(np.float64), P.detach().numpy(), R.detach().numpy())
+    gx = (2.0 / sc.n) * (states[-1].x - target.astype(np.float64))
+    z = lambda *sh: np.zeros(sh, np.float64)  # noqa: E731
+    gl = O.Grads(gx, z(sc.n, 3), z(sc.n, 3, 3), z(sc.n, 3, 3), z(sc.nb, 3), z(sc.nb, 4))
+    _, pg, rg = osim.rollout_grad(states, gl)
+    ((P * torch.tensor(pg)).sum() + (R * torch.tensor(rg)).sum()).backward()
+    g64 = A.grad.numpy()
+    cos = float((g64.ravel() @ g_gpu.ravel()) / (np.linalg.norm(g64) * np.linalg.norm(g_gpu)))
+    print(json.dumps({"particles": sc.n, "substeps": a.env_steps * SUB, "oracle_seconds": time.perf_counter() - t0,
+                      "grad_norm_fp64_oracle": float(np.linalg.norm(g64)), "grad_norm_gpu": float(np.linalg.norm(g_gpu)),
+                      "gpu_vs_fp64_oracle_rel_err": rel_err(g_gpu, g64), "cosine": cos}), flush=True)
 
 
 if __name__ == "__main__":
