@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per particle per kernel launch (state, material and temporaries the
-# kernel must read or write; grid traffic excluded). See the design notes.
+# kernel must read or write; grid traffic excluded), itemised per entry.
 KERNEL_BYTES = {
     "p2g_fwd": 152,      # read x,v,F,C + mass,vol,mu/lam/yield (116) ; write F' (36)
     "grid_fwd": 0,

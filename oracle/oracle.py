@@ -12,8 +12,12 @@ reference's per-substep kernel sequences with numpy buffers:
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may
 import this module; the product (``dexdeform_amd``) never does.
 
-Parity status of each restated function (how it is pinned) is listed in the
-oracle section of the design notes and in ``tests/test_oracle_*.py``.
+Each restated function is pinned by ``tests/test_oracle.py``: closed forms and
+invariants (SVD reconstruction, von Mises projection, SDF distances, mass and momentum
+conservation, exact APIC transfer), numpy and scipy routines (SVD, rotations) and FP64
+central finite differences of every adjoint. Parity unpinned: the exact FP32 bits of the
+reference's NVIDIA build (FMA contraction, atomic order, expf/logf ulps); the
+restatement matches it up to rounding only.
 """
 from __future__ import annotations
 
