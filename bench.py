@@ -13,7 +13,7 @@ the grid update; the recomputing adjoint (the reference's scheme) is measured to
 rollout (forward + backward) of all 64 scenes. Also reported: the same rollout with
 eager launches, and the scenes run one at a time (plain; eager and graph-replayed).
 
-Prints one JSON line (rank 0). --gpus N > 1 is refused: this build runs on one GPU.
+Prints one JSON line (rank 0). --gpus N > 1 is refused: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -383,7 +383,7 @@ def main():
                     help="first run one plain 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not supported (use --gpus 1)",
+        print("bench.py: the benchmark runs on one GPU; multi-GPU runs are not supported (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 
