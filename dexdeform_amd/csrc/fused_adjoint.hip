@@ -193,8 +193,9 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   __shared__ Groups gr;
   __shared__ int s_box[6];
   float *const tx = sh.tile[0], *const ty = sh.tile[1], *const tz = sh.tile[2];
-  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
   const ddm_state cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
@@ -285,8 +286,9 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj_det(ddm_batch B, ddm_det D, con
                                                      const ddm_state *__restrict__ next_tab,
                                                      const ddm_state *__restrict__ cur_grad_tab,
                                                      const ddm_state *__restrict__ next_grad_tab) {
-  if ((int)blockIdx.x >= live_chunks(B)) return;
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const int p = ch.y + (int)threadIdx.x;
   if (p >= ch.z) return;
   const ddm_scene &S = B.scenes[ch.x];
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj_det(ddm_batch B, ddm_det D, con
   if (B.options & DDM_GRAD_OVERWRITE) st3(cur_grad.x + 3 * (size_t)p, gxp);
   else add3(cur_grad.x + 3 * (size_t)p, gxp);
   // at the particle's sorted position (the P2G pass of this adjoint sorted the same bins)
-  const size_t slot = (size_t)blockIdx.x * kWG + threadIdx.x;
+  const size_t slot = (size_t)ci * kWG + threadIdx.x;
   float4 *r = reinterpret_cast<float4 *>(D.sorted) + 4 * (size_t)D.key_slot[2 * slot + 1];
   r[0] = make_float4(st.fx[0], st.fx[1], st.fx[2], 0.f);
   r[1] = make_float4(g_nv.x, g_nv.y, g_nv.z, gC.a[0]);
@@ -350,8 +352,9 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
-  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], cur_grad = cur_grad_tab[ch.x], next_grad = next_grad_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;

