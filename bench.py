@@ -47,11 +47,11 @@ KERNEL_BYTES = {
 # for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0). Re-measured whenever a
 # kernel changes; reported as roofline.traffic for the default workload only.
 TRAFFIC_BYTES = {
-    "p2g_fwd": 275, "grid_fwd": 10, "g2p_fwd": 88, "p2g_recompute": 316, "grid_recompute": 7, "g2p_adj": 163,
-    "grid_adj": 18, "p2g_adj": 462,
+    "p2g_fwd": 272, "grid_fwd": 10, "g2p_fwd": 75, "p2g_recompute": 311, "grid_recompute": 7, "g2p_adj": 148,
+    "grid_adj": 18, "p2g_adj": 442,
     # taped rollout
-    "taped_p2g_fwd": 465, "taped_grid_fwd": 17, "taped_g2p_fwd": 88, "taped_restore": 13, "taped_g2p_adj": 163,
-    "taped_grid_adj": 18, "taped_p2g_adj": 462,
+    "taped_p2g_fwd": 461, "taped_grid_fwd": 17, "taped_g2p_fwd": 75, "taped_restore": 13, "taped_g2p_adj": 148,
+    "taped_grid_adj": 18, "taped_p2g_adj": 442,
 }
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
