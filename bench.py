@@ -429,6 +429,27 @@ def main():
     if tape:
         log(f"taped adjoint: {ro.tape_recomputed()} substeps of the last segment recomputed; "
             f"tape {ro.tape.nbytes() / 2**30:.1f} GiB")
+    # the same replayed rollout with its inputs handed over in pageable host memory (as a
+    # caller holding numpy arrays would) and its results read back: initial positions and
+    # poses uploaded, final positions and the initial-position gradient downloaded
+    x0_host = np.concatenate([s.x0 for s in scenes])
+    pos_host = np.ascontiguousarray(np.concatenate([p[0] for p in poses], 1))
+    rot_host = np.ascontiguousarray(np.concatenate([p[1] for p in poses], 1))
+    pcie_bytes = x0_host.nbytes + pos_host.nbytes + rot_host.nbytes + 2 * x0_host.nbytes
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ro.set_initial(torch.from_numpy(x0_host))
+        if eng.n_bodies_total:
+            ro.set_poses(torch.from_numpy(pos_host), torch.from_numpy(rot_host))
+        ro.replay()
+        x_back, g_back = ro.final_state().x.cpu(), ro.initial_grad().x.cpu()
+    torch.cuda.synchronize()
+    pcie_sec = (time.perf_counter() - t0) / args.steps
+    pcie_value = n_total * T / pcie_sec
+    log(f"batched graph with host inputs and outputs: {pcie_value:.3e} particle-substeps/s "
+        f"({pcie_bytes / 1e6:.0f} MB over PCIe per rollout, {pcie_sec * 1e3 - ms_step:.1f} ms more)")
+    del x_back, g_back
     recompute_value = None
     if tape and not args.no_recompute:
         # the same rollout with the reference's recompute-in-backward adjoint
@@ -529,6 +550,8 @@ def main():
         "recompute_value": recompute_value,
         "deterministic_value": det_value,
         "eager_value": eager_value,
+        "pcie_inclusive_value": pcie_value,
+        "pcie_bytes_per_rollout": pcie_bytes,
         "plain_value": plain_value,
         "plain_eager_value": plain_eager_value,
         "batched_over_plain": (value / plain_value) if plain_value else None,
