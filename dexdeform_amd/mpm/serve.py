@@ -202,62 +202,82 @@ class _TickTables:
     """Per-tick launch tables at fixed device addresses, so that a tick's launches can be
     captured once in a HIP graph and replayed with each tick's contents.
 
-    Layout (parts 256-byte aligned): for the forward launch and then the adjoint launch,
-    the scene records [max_scenes], the state records (current, next; for the adjoint
-    also their gradients) [max_scenes] and the live counts (chunks, scenes); then each
-    launch's chunk table [max_chunks]. A tick's contents are packed into one slot of a
-    pinned host ring and copied to the device in one asynchronous transfer on the
-    launch stream; a host slot is reused only after the copy that read it completed.
-    Launches on the same stream consume the device tables in order, so one device copy
-    suffices."""
+    A tick's key -- (forward scenes, forward chunks, adjoint scenes, adjoint chunks), the
+    launch sizes, which the live counts may undercut -- fixes a compact layout: for the
+    forward launch, then the adjoint launch, the scene records, the state records (current,
+    next; for the adjoint also their gradients), the live counts (chunks, scenes) and the
+    chunk table, each 64-byte aligned and sized for the key. A tick's contents are packed
+    into one slot of a pinned host ring at its key's offsets and copied to the device in
+    one asynchronous transfer of the layout's span (about 1.4 KB for one request of up to
+    64 chunks) on the launch stream; a host slot is reused only after the copy that read it
+    completed. Launches on the same stream consume the device tables in order, so one
+    device region suffices."""
 
-    ALIGN = 256
-    PARTS = {"F": ("recs", "cur", "nxt", "live"), "B": ("recs", "cur", "nxt", "gcur", "gnxt", "live")}
+    ALIGN = 64
+    PARTS = {"F": ("recs", "cur", "nxt", "live", "chunks"),
+             "B": ("recs", "cur", "nxt", "gcur", "gnxt", "live", "chunks")}
 
     def __init__(self, device, max_scenes, max_chunks, slots=8):
         self.device = device
         self.max_scenes = int(max_scenes)
         self.max_chunks = int(max_chunks)
-        a = lambda n: (n + self.ALIGN - 1) // self.ALIGN * self.ALIGN
-        size = {"recs": _lib.SCENE_DTYPE.itemsize * self.max_scenes, "cur": 48 * self.max_scenes,
-                "nxt": 48 * self.max_scenes, "gcur": 48 * self.max_scenes, "gnxt": 48 * self.max_scenes,
-                "live": 8, "chunks": 16 * self.max_chunks}
-        self.off, off = {}, 0
-        for kind in ("F", "B"):
-            for part in self.PARTS[kind]:
-                self.off[kind, part] = off
-                off += a(size[part])
-        for kind in ("F", "B"):
-            self.off[kind, "chunks"] = off
-            off += a(size["chunks"])
-        self.size = size
-        self.nbytes = off
+        self.nbytes = self._offsets((self.max_scenes, self.max_chunks, self.max_scenes, self.max_chunks))[1]
         self.dev = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.base = self.dev.data_ptr()
         self.host = [torch.empty(self.nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
         self.events = [None] * slots
         self.i = 0
-        base = self.dev.data_ptr()
-        self.addr = {k: base + o for k, o in self.off.items()}
+        self._layouts = {}
 
-    def fits(self, n_scenes, n_chunks):
-        return n_scenes <= self.max_scenes and n_chunks <= self.max_chunks
+    @staticmethod
+    def _part_bytes(part, scenes, chunks):
+        if part == "recs":
+            return _lib.SCENE_DTYPE.itemsize * scenes
+        if part == "live":
+            return 8
+        if part == "chunks":
+            return 16 * chunks
+        return 48 * scenes  # a ddm_state record per scene
 
-    def stage(self, contents):
-        """contents: {(kind, part): numpy array}; packed at the fixed offsets and copied."""
+    @classmethod
+    def _offsets(cls, key):
+        fs, fc, bs, bc = key
+        off, cap, o = {}, {}, 0
+        for kind, sc, ch in (("F", fs, fc), ("B", bs, bc)):
+            if not sc:
+                continue
+            for part in cls.PARTS[kind]:
+                off[kind, part] = o
+                cap[kind, part] = cls._part_bytes(part, sc, ch)
+                o += (cap[kind, part] + cls.ALIGN - 1) // cls.ALIGN * cls.ALIGN
+        return off, o, cap
+
+    def fits(self, key) -> bool:
+        return self._offsets(key)[1] <= self.nbytes
+
+    def layout(self, key):
+        """(device addresses, offsets, span, part capacities) of a key's tables."""
+        lay = self._layouts.get(key)
+        if lay is None:
+            off, span, cap = self._offsets(key)
+            lay = ({k: self.base + o for k, o in off.items()}, off, span, cap)
+            self._layouts[key] = lay
+        return lay
+
+    def stage(self, key, contents):
+        """contents: {(kind, part): numpy array}; packed at the key's offsets and copied."""
+        _, off, span, cap = self.layout(key)
         slot = self.i
         self.i = (self.i + 1) % len(self.host)
         if self.events[slot] is not None:
             self.events[slot].synchronize()
         h = self.host[slot].numpy()
-        end = 0
-        for key, arr in contents.items():
+        for k, arr in contents.items():
             b = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
-            o = self.off[key]
-            if b.size > self.size[key[1]]:
-                raise ValueError(f"{key} table exceeds its capacity")
-            h[o:o + b.size] = b
-            end = max(end, o + b.size)
-        self.dev[:end].copy_(self.host[slot][:end], non_blocking=True)
+            if b.size > cap[k]:
+                raise ValueError(f"{k} table exceeds its capacity")
+            h[off[k]:off[k] + b.size] = b
+        self.dev[:span].copy_(self.host[slot][:span], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record()
         self.events[slot] = ev
@@ -595,12 +615,11 @@ class RolloutServer:
             c[kind, "gnxt"] = _state_records([a.grad_record(op[4], op[2] + 1) for a, op in entries])
         return c
 
-    def _launch(self, kind, n_scenes, n_chunks, live):
+    def _launch(self, kind, n_scenes, n_chunks, live, A):
         """One fused launch sequence over the staged tables of `kind` ("F": forward-type
-        substeps, "B": adjoint substeps). With `live`, the launch is sized for n_scenes /
-        n_chunks and its kernels read the counts actually used from the staged tables
-        (what a captured tick replays); without, it is sized exactly."""
-        A = self.tables.addr
+        substeps, "B": adjoint substeps) at addresses A. With `live`, the launch is sized
+        for n_scenes / n_chunks and its kernels read the counts actually used from the
+        staged tables (what a captured tick replays); without, it is sized exactly."""
         batch = _lib.DdmBatch(n_scenes, n_chunks, self.arena.n, _lib.DDM_MAX_BODIES, A[kind, "recs"],
                               A[kind, "chunks"], self.grid_mmv.data_ptr(), self.grid_v.data_ptr(),
                               self.grid_dv.data_ptr(), self.grid_dmmv.data_ptr(), self.block_flags.data_ptr(),
@@ -618,10 +637,11 @@ class RolloutServer:
     def _launch_tick(self, key, live):
         """key: (forward scenes, forward chunks, adjoint scenes, adjoint chunks); 0 = no launch."""
         fs, fc, bs, bc = key
+        A = self.tables.layout(key)[0]
         if fs:
-            self._launch("F", fs, fc, live)
+            self._launch("F", fs, fc, live, A)
         if bs:
-            self._launch("B", bs, bc, live)
+            self._launch("B", bs, bc, live, A)
 
     def _run_tick(self, fwd, bwd):
         contents = {}
@@ -629,22 +649,25 @@ class RolloutServer:
             contents.update(self._contents(fwd, "F"))
         if bwd:
             contents.update(self._contents(bwd, "B"))
-        n_chunks = max(len(contents[k, "chunks"]) if (k, "chunks") in contents else 0 for k in "FB")
-        if not self.tables.fits(max(len(fwd), len(bwd)), n_chunks):
+        nf, nb = len(fwd), len(bwd)
+        cf = len(contents["F", "chunks"]) if fwd else 0
+        cb = len(contents["B", "chunks"]) if bwd else 0
+        if self.use_graphs:
+            # launch sizes of the captured tick: scene and chunk counts rounded up to powers
+            # of two (one graph per key); the live counts carry the exact ones
+            key = (_bucket(nf, 1) if fwd else 0, _bucket(cf, 64) if fwd else 0,
+                   _bucket(nb, 1) if bwd else 0, _bucket(cb, 64) if bwd else 0)
+        else:
+            key = (nf, cf, nb, cb)
+        if not self.tables.fits(key):
             torch.cuda.synchronize()  # the old tables may still be read by queued launches
-            self.tables = _TickTables(self.device, max(self.tables.max_scenes, len(fwd), len(bwd)),
-                                      _bucket(n_chunks, self.tables.max_chunks))
+            self.tables = _TickTables(self.device, _bucket(max(self.tables.max_scenes, nf, nb), 1),
+                                      _bucket(max(self.tables.max_chunks, cf, cb), 64))
             self.graphs.clear()
-        self.tables.stage(contents)
+        self.tables.stage(key, contents)
         if not self.use_graphs:
-            exact = (len(fwd), len(contents.get(("F", "chunks"), ())), len(bwd), len(contents.get(("B", "chunks"), ())))
-            self._launch_tick(exact, live=False)
+            self._launch_tick(key, live=False)
             return
-        # launch sizes of the captured tick: every scene slot of the tables, and the chunk
-        # count rounded up to a power of two (one graph per pair of chunk buckets)
-        cap_s, cap_c = self.tables.max_scenes, self.tables.max_chunks
-        key = (cap_s if fwd else 0, min(_bucket(len(contents["F", "chunks"]), 64), cap_c) if fwd else 0,
-               cap_s if bwd else 0, min(_bucket(len(contents["B", "chunks"]), 64), cap_c) if bwd else 0)
         g = self.graphs.get(key)
         if g is None:
             # first tick of this shape: capture its launches on a side stream (capture runs

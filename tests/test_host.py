@@ -217,3 +217,28 @@ def test_tick_graph_buckets():
     from dexdeform_amd.mpm.serve import _bucket
     assert _bucket(1, 64) == 64 and _bucket(64, 64) == 64 and _bucket(65, 64) == 128
     assert _bucket(1000, 64) == 1024 and _bucket(3, 4) == 4 and _bucket(5, 4) == 8
+
+
+def test_tick_table_layout_is_compact_aligned_and_disjoint():
+    """A tick key's tables: every part 64-byte aligned, parts disjoint and in order,
+    capacities as the launch sizes need (records 176 B and state records 48 B per scene,
+    16 B per chunk), and the span a few KB for one small request."""
+    from dexdeform_amd import _lib
+    from dexdeform_amd.mpm.serve import _TickTables
+    for key in [(1, 64, 0, 0), (0, 0, 1, 64), (1, 64, 1, 64), (4, 256, 2, 128), (256, 4096, 256, 4096)]:
+        off, span, cap = _TickTables._offsets(key)
+        parts = sorted(off.items(), key=lambda kv: kv[1])
+        for (k, o), nxt in zip(parts, parts[1:] + [(None, span)]):
+            assert o % 64 == 0
+            assert o + cap[k] <= nxt[1]
+        fs, fc, bs, bc = key
+        for kind, sc, ch in (("F", fs, fc), ("B", bs, bc)):
+            if not sc:
+                assert not any(k[0] == kind for k in off)
+                continue
+            assert cap[kind, "recs"] == _lib.SCENE_DTYPE.itemsize * sc
+            assert cap[kind, "cur"] == cap[kind, "nxt"] == 48 * sc
+            assert cap[kind, "chunks"] == 16 * ch and cap[kind, "live"] == 8
+            if kind == "B":
+                assert cap["B", "gcur"] == cap["B", "gnxt"] == 48 * sc
+    assert _TickTables._offsets((1, 64, 1, 64))[1] < 4096
