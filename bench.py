@@ -133,7 +133,7 @@ def serve_benchmark(dev, rates, seconds, seed, graphs=True):
     return out
 
 
-def named_rollouts(dev, steps, large=True):
+def named_rollouts(dev, steps, large=True, only=None):
     """The single-scene workloads, each one forward + backward rollout through the
     single-scene path (taped adjoint): the plain 10k-particle scene (50 substeps, two
     capsules with constant velocities), the 50k-particle hand-contact scene (22 links,
@@ -175,16 +175,19 @@ def named_rollouts(dev, steps, large=True):
         del ro, eng
         torch.cuda.empty_cache()
 
-    sc = W.plain_scene(n_particles=10000)
-    T = 50
-    vel = np.random.Generator(np.random.PCG64(1)).uniform(-0.5, 0.5, size=(sc.nb, 3)).astype(np.float32)
-    t = np.arange(T + 1, dtype=np.float32)[:, None, None] * sc.spec.dt
-    run("plain_10k", sc, T, None, (torch.tensor(sc.body_pos[None] + t * vel[None], device=dev),
-                                    torch.tensor(np.repeat(sc.body_rot[None], T + 1, 0), device=dev)), True)
-    sc, tree, q0 = W.hand_scene(50000)
-    P, R = tree.to(device=dev).trajectory(q0.to(dev), torch.tensor(W.hand_actions(20), device=dev), 10)
-    run("hand_50k", sc, 200, 10, (P.detach(), R.detach()), True)
-    if large:
+    want = lambda name: only is None or name in only  # noqa: E731
+    if want("plain_10k"):
+        sc = W.plain_scene(n_particles=10000)
+        T = 50
+        vel = np.random.Generator(np.random.PCG64(1)).uniform(-0.5, 0.5, size=(sc.nb, 3)).astype(np.float32)
+        t = np.arange(T + 1, dtype=np.float32)[:, None, None] * sc.spec.dt
+        run("plain_10k", sc, T, None, (torch.tensor(sc.body_pos[None] + t * vel[None], device=dev),
+                                        torch.tensor(np.repeat(sc.body_rot[None], T + 1, 0), device=dev)), True)
+    if want("hand_50k"):
+        sc, tree, q0 = W.hand_scene(50000)
+        P, R = tree.to(device=dev).trajectory(q0.to(dev), torch.tensor(W.hand_actions(20), device=dev), 10)
+        run("hand_50k", sc, 200, 10, (P.detach(), R.detach()), True)
+    if large and want("large_1m"):
         sc, tree, q0 = W.large_scene(1_000_000)
         P, R = tree.to(device=dev).trajectory(q0.to(dev), torch.tensor(W.hand_actions(50), device=dev), 10)
         run("large_1m", sc, 500, 25, (P.detach(), R.detach()), False)
@@ -379,6 +382,9 @@ def main():
                     help="skip the single-scene plain 10k / hand 50k / large 1M rollouts")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="arrival window per rate: rate x seconds requests (at least 24)")
+    ap.add_argument("--named-only", default=None, metavar="NAMES",
+                    help="diagnostic: run only these single-scene rollouts (comma list of plain_10k, "
+                         "hand_50k, large_1m) and print their figures")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -389,6 +395,10 @@ def main():
 
     import torch
     dev = torch.device("cuda:0")
+    if args.named_only:  # diagnostic: only the named single-scene rollouts, one JSON line of them
+        print(json.dumps({"single_scene_rollouts": named_rollouts(dev, max(1, args.steps),
+                                                                  only=args.named_only.split(","))}), flush=True)
+        return
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
