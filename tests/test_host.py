@@ -242,3 +242,18 @@ def test_tick_table_layout_is_compact_aligned_and_disjoint():
             if kind == "B":
                 assert cap["B", "gcur"] == cap["B", "gnxt"] == 48 * sc
     assert _TickTables._offsets((1, 64, 1, 64))[1] < 4096
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus N > 1 (or a multi-rank launch) exits with status 2
+    and says so, before touching a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for extra_env, args in (({}, ["--gpus", "2"]), ({"WORLD_SIZE": "2"}, [])):
+        env = dict(os.environ, **extra_env)
+        r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), *args], capture_output=True, text=True,
+                           env=env, timeout=120)
+        assert r.returncode == 2
+        assert "one GPU" in r.stderr
