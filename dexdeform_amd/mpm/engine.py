@@ -281,7 +281,10 @@ class Engine:
             self._rebuild_batch()
 
     def _rebuild_batch(self):
+        """New launch record. Bumps ``generation``: launch tables and chunk tensors of
+        earlier generations may be freed, so graphs captured before must not be replayed."""
         p = _lib.ptr
+        self.generation = getattr(self, "generation", 0) + 1
         self._batch = _lib.DdmBatch(
             len(self.scenes), self.n_chunks, self.n_blocks, max(s.n_bodies for s in self.scenes),
             p(self.scene_table), p(self.chunks), p(self.grid_mmv), p(self.grid_v), p(self.grid_dv), p(self.grid_dmmv),

@@ -139,8 +139,8 @@ class HingeTree:
     """
 
     def __init__(self, parents, offsets, axes, geom_offsets, lower, upper, base, scale, action_map=None,
-                 device="cpu"):
-        f = lambda a: torch.as_tensor(a, dtype=torch.float32, device=device)
+                 device="cpu", dtype=torch.float32):
+        f = lambda a: torch.as_tensor(a, dtype=dtype, device=device)
         self.parents = list(int(p) for p in parents)
         self.n = len(self.parents)
         assert all(p < i for i, p in enumerate(self.parents))

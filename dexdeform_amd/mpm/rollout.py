@@ -228,12 +228,16 @@ class Rollout:
             self.forward()
             self.backward(x_grad=x_grad)
         self._graph = g
+        self._graph_generation = self.engine.generation
         return g
 
     def replay(self):
         """One captured forward + backward pass (see capture)."""
         if getattr(self, "_graph", None) is None:
             raise RuntimeError("no captured graph: call capture() first")
+        if self._graph_generation != self.engine.generation:
+            raise RuntimeError("the engine's launch tables changed since capture (set_counts or a new batch): "
+                               "capture again")
         self._graph.replay()
 
     def state_bytes(self) -> int:

@@ -552,6 +552,13 @@ class RolloutServer:
 
     # ------------------------------------------------------------------ admission
     def submit(self, req: Request):
+        if req.T < 1:
+            raise ValueError(f"request {req.rid}: the horizon must be at least one substep")
+        if req.checkpoint_every is not None and (req.checkpoint_every < 1 or req.T % req.checkpoint_every):
+            raise ValueError(f"request {req.rid}: checkpoint_every={req.checkpoint_every} must divide "
+                             f"the horizon {req.T}")
+        if len(req.pose_rot) != len(req.pose_pos) or len(req.target) != req.n:
+            raise ValueError(f"request {req.rid}: pose or target arrays do not match the horizon / particles")
         if request_bytes(req) > self.memory_budget:
             raise ValueError(f"request {req.rid} needs more memory than the whole budget")
         gd = req.spec.grid_dim
@@ -590,10 +597,17 @@ class RolloutServer:
             b0 = self.arena.alloc(nblk)
             if b0 is None:
                 break
+            t0 = time.perf_counter()
+            try:
+                act = _Active(self, req, b0, now, K)
+            except Exception:
+                # nothing was committed yet: give the blocks back and leave the request
+                # queued for the caller to inspect (the running requests are unaffected)
+                self.arena.release(b0, nblk)
+                raise
             self.waiting.pop(0)
             self.used_bytes += need
-            t0 = time.perf_counter()
-            self.active.append(_Active(self, req, b0, now, K))
+            self.active.append(act)
             self.host_s["admit"] += time.perf_counter() - t0
 
     # ------------------------------------------------------------------ launches

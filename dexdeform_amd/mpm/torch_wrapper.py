@@ -35,6 +35,7 @@ class GradModel:
         self.tape = tape
         self._tapes = None
         self._graphs = {}
+        self._graph_generation = None
         self._pose_in = None
         self._pose_grad_out = None
         self.env = env
@@ -124,7 +125,13 @@ class GradModel:
         if not self.graphs or not sim.device.type == "cuda":
             fn()
             return
-        key = (kind, f, id(sim.engine._batch)) + tuple(id(s) for s in sim.states[f:f + self.substeps + 1])
+        gen = sim.engine.generation
+        if gen != self._graph_generation:
+            # the engine rebuilt its launch record (e.g. a new particle count): graphs
+            # captured before point at freed chunk tables and hold stale sizes
+            self._graphs.clear()
+            self._graph_generation = gen
+        key = (kind, f) + tuple(id(s) for s in sim.states[f:f + self.substeps + 1])
         g = self._graphs.get(key)
         if g is not None:
             g.replay()

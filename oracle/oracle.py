@@ -154,6 +154,26 @@ def xform(pos, rot, pts, inverse=False, dtype=np.float64):
     return out
 
 
+def grid_op(grid_m, grid_mv, grid_dim, body_pos, body_rot, next_pos, next_rot, tfsr, args, dx, dt, gravity,
+            ground_friction=0.0, out_fill=0.0, dtype=np.float64):
+    """One call of the grid update (integrator.cu:647-777) on a dense x-major grid.
+    Returns (grid_v_out [G,3], tape [G, nb+1, 3]); nodes the update skips (m <= 1e-12)
+    keep ``out_fill``. The tape holds the node velocity entering each body's contact,
+    then the velocity after the last body (the reference's grid_body_v_in layout)."""
+    G = int(np.prod(grid_dim))
+    nb = len(tfsr)
+    f = lambda a, w: np.ascontiguousarray(a, dtype=dtype).reshape(-1, w) if nb else np.zeros((1, w), dtype)
+    m = np.ascontiguousarray(grid_m, dtype=dtype).reshape(G)
+    mv = np.ascontiguousarray(grid_mv, dtype=dtype).reshape(G, 3)
+    out = np.full((G, 3), out_fill, dtype)
+    tape = np.zeros((G, nb + 1, 3), dtype)
+    grid = np.ascontiguousarray(grid_dim, dtype=np.int32)
+    _call("grid_op", dtype, m, mv, tape, f(body_pos, 3), f(body_rot, 4), f(next_pos, 3), f(next_rot, 4),
+          f(tfsr, 4), f(args, 4), out, grid, nb, _F(dx), _F(dt), _F(ground_friction),
+          np.ascontiguousarray(gravity, dtype=dtype))
+    return out, tape
+
+
 # ------------------------------------------------------------------------ scene + simulator
 @dataclass
 class Scene:

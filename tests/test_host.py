@@ -99,6 +99,42 @@ def test_hinge_tree_kinematics_and_gradients():
     assert a.grad is not None and torch.isfinite(a.grad).all()
 
 
+def test_hinge_tree_fk_two_link_arm_closed_form():
+    """Planar two-link arm (hinges about z, link lengths L1, L2) under a rotated, shifted
+    base: the tip of link 2 sits at p + Rb (L1 (cos a, sin a, 0) + L2 (cos(a+b), sin(a+b), 0))
+    and its orientation is Rb Rz(a + b) (FK order T_parent @ Trans(offset) @ Rot(axis, q)
+    @ geom_offset, mpm/hand.py:347-381)."""
+    from scipy.spatial.transform import Rotation
+    L1, L2 = 0.07, 0.05
+    base = np.eye(4)
+    Rb = Rotation.from_rotvec([0.4, -0.3, 0.9]).as_matrix()
+    base[:3, :3] = Rb
+    base[:3, 3] = [0.5, 0.3, 0.45]
+    geoms = np.tile(np.eye(4), (2, 1, 1))
+    geoms[0, 0, 3] = L1 / 2   # link-1 geom at its midpoint
+    geoms[1, 0, 3] = L2       # link-2 geom at its tip
+    tree = K.HingeTree([-1, 0], [[0, 0, 0], [L1, 0, 0]], [[0, 0, 1], [0, 0, 1]], geoms, [-3, -3], [3, 3],
+                       base, [1.0, 1.0], dtype=torch.float64)
+    for a, b in ((0.3, -0.8), (1.2, 0.5), (-2.0, 2.5)):
+        pos, quat = tree.fk(torch.tensor([a, b], dtype=torch.float64))
+        p0 = base[:3, 3] + Rb @ (L1 / 2 * np.array([np.cos(a), np.sin(a), 0]))
+        p1 = base[:3, 3] + Rb @ (L1 * np.array([np.cos(a), np.sin(a), 0]) +
+                                 L2 * np.array([np.cos(a + b), np.sin(a + b), 0]))
+        assert np.allclose(pos[0].numpy(), p0, rtol=0, atol=1e-12)
+        assert np.allclose(pos[1].numpy(), p1, rtol=0, atol=1e-12)
+        for link, ang in ((0, a), (1, a + b)):
+            R = Rb @ Rotation.from_rotvec([0, 0, ang]).as_matrix()
+            x, y, z, w = Rotation.from_matrix(R).as_quat()
+            qe = np.array([w, x, y, z])
+            qg = quat[link].numpy()
+            assert min(np.abs(qg - qe).max(), np.abs(qg + qe).max()) < 1e-12
+    # the joint ramp feeds the same FK: substep i of action u sees q0 + u * scale * (i + 1)
+    q0 = torch.tensor([0.1, 0.2], dtype=torch.float64)
+    pos, rot, qe = tree.poses(q0, torch.tensor([0.5, -0.25], dtype=torch.float64), 3)
+    assert torch.allclose(qe, q0 + torch.tensor([0.5, -0.25], dtype=torch.float64) * 3)
+    assert torch.allclose(pos[1], tree.fk(q0 + torch.tensor([0.5, -0.25], dtype=torch.float64) * 2)[0])
+
+
 def test_chunk_table_covers_every_particle_once():
     counts = [1, 255, 256, 257, 1000, 0]
     tab = chunk_table(counts)
@@ -257,3 +293,32 @@ def test_bench_refuses_more_than_one_gpu():
                            env=env, timeout=120)
         assert r.returncode == 2
         assert "one GPU" in r.stderr
+
+
+def test_server_submit_validates_and_admission_rolls_back(monkeypatch):
+    """submit() rejects a checkpoint interval that does not divide the horizon (and empty
+    horizons) before queueing; if building a request's device state fails at admission,
+    its arena blocks and memory are given back and the request stays queued."""
+    from dexdeform_amd.mpm import serve as S
+    sc = W.plain_scene(n_particles=500, grid=32)
+    T = 12
+    pos = np.repeat(sc.body_pos[None], T + 1, 0)
+    rot = np.repeat(sc.body_rot[None], T + 1, 0)
+    srv = S.RolloutServer.__new__(S.RolloutServer)  # host bookkeeping only, no device
+    srv.memory_budget, srv.full_tape_bytes = 1 << 30, 0
+    srv.arena, srv.waiting, srv.active, srv.max_active = S.BlockArena(4096), [], [], 8
+    srv.used_bytes, srv.host_s = 0, {"admit": 0.0}
+    for bad in (5, 0, -2):
+        with pytest.raises(ValueError):
+            srv.submit(S.Request(0, sc.spec, sc.x0, pos, rot, sc.x0, checkpoint_every=bad))
+    with pytest.raises(ValueError):
+        srv.submit(S.Request(1, sc.spec, sc.x0, pos[:1], rot[:1], sc.x0))
+    assert srv.waiting == []
+    srv.submit(S.Request(2, sc.spec, sc.x0, pos, rot, sc.x0, checkpoint_every=4))
+
+    def boom(*a, **k):
+        raise RuntimeError("device allocation failed")
+    monkeypatch.setattr(S, "_Active", boom)
+    with pytest.raises(RuntimeError):
+        srv._admit(now=1.0)
+    assert srv.used_bytes == 0 and srv.arena.free == [(0, 4096)] and len(srv.waiting) == 1 and srv.active == []

@@ -377,3 +377,209 @@ def test_fp32_oracle_within_rounding_of_fp64():
     s32 = O.OracleSim(sc, np.float32).substep(st, npos, nrot)
     for a, b in zip((s32.x, s32.v, s32.F, s32.C), (s64.x, s64.v, s64.F, s64.C)):
         assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+
+
+# ------------------------------------------------------------------ grid update forward (cu:647-777)
+# Closed-form cases: every expected value below comes from the geometry the test builds
+# (node position, body pose, body-frame point, face normal), not from the oracle's SDFs.
+GD = (16, 16, 16)
+GDX = 1.0 / 16
+GDT = 1e-3
+GRAV = np.array([0.3, -2.0, 0.1])
+
+
+def _gidx(i, j, k):
+    return (i * GD[1] + j) * GD[2] + k
+
+
+def _rotq(axis, angle):
+    """(w, x, y, z) quaternion and matrix of a rotation, from scipy."""
+    from scipy.spatial.transform import Rotation
+    r = Rotation.from_rotvec(np.asarray(axis, float) / np.linalg.norm(axis) * angle)
+    x, y, z, w = r.as_quat()
+    return np.array([w, x, y, z]), r.as_matrix()
+
+
+def _one_node(node, v_in, bodies, gf=0.0, m=2.0):
+    """Grid with one active node whose velocity after gravity is v_in. bodies: list of
+    (tfsr, args, pos, quat, next_pos, next_quat)."""
+    G = int(np.prod(GD))
+    gm = np.zeros(G)
+    gmv = np.zeros((G, 3))
+    t = _gidx(*node)
+    gm[t] = m
+    gmv[t] = (np.asarray(v_in) - GDT * GRAV) * m
+    cols = list(zip(*bodies)) if bodies else [[]] * 6
+    out, tape = O.grid_op(gm, gmv, GD, cols[2], cols[3], cols[4], cols[5], cols[0], cols[1], GDX, GDT, GRAV,
+                          ground_friction=gf, out_fill=7.0)
+    return out[t], tape[t], out, t
+
+
+def _placed(node, R, q, x_b, next_R=None, next_q=None, shift=(0, 0, 0)):
+    """Body pose putting grid node `node` at body-frame point x_b; optionally a next pose."""
+    xw = np.asarray(node, float) * GDX
+    p = xw - R @ x_b
+    if next_R is None:
+        return p, q, p, q
+    return p, q, p + np.asarray(shift), next_q
+
+
+def test_grid_op_free_node_and_activity_threshold():
+    G = int(np.prod(GD))
+    gm = np.zeros(G)
+    gmv = np.zeros((G, 3))
+    a, b, c = _gidx(8, 8, 8), _gidx(8, 9, 8), _gidx(8, 10, 8)
+    gm[a], gmv[a] = 2.0, [0.4, -0.2, 0.6]
+    gm[b], gmv[b] = 1e-13, [1.0, 1.0, 1.0]      # below the 1e-12 activity threshold: skipped
+    gm[c], gmv[c] = 4e-12, [4e-12, 0.0, 0.0]    # just above it: updated
+    far_box = ([0, 0.0, 666, 0], [0.02, 0.02, 0.02, 0], [0.1, 0.1, 0.1], [1, 0, 0, 0])
+    out, tape = O.grid_op(gm, gmv, GD, [far_box[2]], [far_box[3]], [far_box[2]], [far_box[3]], [far_box[0]],
+                          [far_box[1]], GDX, GDT, GRAV, out_fill=7.0)
+    expect = np.array([0.2, -0.1, 0.3]) + GDT * GRAV
+    assert np.allclose(out[a], expect, rtol=0, atol=1e-15)
+    assert np.all(out[b] == 7.0)
+    assert np.allclose(out[c], np.array([1.0, 0, 0]) + GDT * GRAV, atol=1e-12)
+    # tape: velocity entering body 0, then after the last body (no contact here: equal)
+    assert np.allclose(tape[a, 0], expect, atol=1e-15) and np.allclose(tape[a, 1], expect, atol=1e-15)
+    untouched = np.ones(G, bool)
+    untouched[[a, c]] = False
+    assert np.all(out[untouched] == 7.0)
+
+
+@pytest.mark.parametrize("approach", [True, False])
+def test_grid_op_static_rotated_box_frictionless(approach):
+    """Node inside a static rotated box (influence 1), friction 0: the inward normal
+    component is removed, a separating velocity passes through unchanged."""
+    half = np.array([0.05, 0.04, 0.06])
+    q, R = _rotq([1, 2, 3], 0.7)
+    x_b = np.array([0.3, 0.8, -0.2]) * half  # nearest face: +y of the body
+    n = R @ np.array([0.0, 1.0, 0.0])
+    t = np.cross(n, [1.0, 0, 0])
+    t *= 0.3 / np.linalg.norm(t)
+    v = (-0.7 if approach else 0.5) * n + t
+    node = (8, 8, 8)
+    p, q0, pn, qn = _placed(node, R, q, x_b)
+    far = ([1, 0.0, 666, 0], [0.01, 0.02, 0, 0], [0.9, 0.9, 0.9], [1, 0, 0, 0], [0.9, 0.9, 0.9], [1, 0, 0, 0])
+    box = ([0, 0.0, 666, 0], [*half, 0], p, q0, pn, qn)
+    v_out, tape, _, _ = _one_node(node, v, [box, far])
+    expect = t if approach else v
+    assert np.allclose(v_out, expect, atol=1e-10)
+    assert np.allclose(tape[0], v, atol=1e-12)      # entering body 0
+    assert np.allclose(tape[1], expect, atol=1e-10)  # entering body 1 = after body 0
+    assert np.allclose(tape[2], expect, atol=1e-10)  # after the last body
+
+
+def _capsule_case(mu, v_rel_n, v_rel_t, moving=True, round_=0.0):
+    r, h = 0.03, 0.1
+    q, R = _rotq([0.2, -1, 0.4], 0.9)
+    x_b = np.array([0.01, 0.02, 0.005])        # inside: segment point (0, 0.02, 0), radial (0.01, 0, 0.005)
+    nb_ = np.array([0.01, 0.0, 0.005]) / np.hypot(0.01, 0.005)
+    node = (7, 9, 8)
+    xw = np.asarray(node, float) * GDX
+    if moving:
+        q1, R1 = _rotq([1, 0.5, -0.3], 0.02)
+        q1 = np.array([q1[0] * q[0] - q1[1:] @ q[1:], *(q1[0] * q[1:] + q[0] * q1[1:] + np.cross(q1[1:], q[1:]))])
+        R1 = R1 @ R
+        shift = np.array([1e-4, -2e-4, 5e-5])
+    else:
+        q1, R1, shift = q, R, np.zeros(3)
+    p = xw - R @ x_b
+    pn = p + shift
+    bv = (pn + R1 @ x_b - xw) / GDT             # body velocity at the node
+    n = R @ nb_
+    t = np.cross(n, [0, 0, 1.0])
+    t /= np.linalg.norm(t)
+    v = bv + v_rel_n * n + v_rel_t * t
+    cap = ([1, mu, 666, round_], [r, h, 0, 0], p, q, pn, q1)
+    v_out, _, _, _ = _one_node(node, v, [cap])
+    return v_out, bv, n, t
+
+
+def test_grid_op_moving_capsule_sticks_at_high_friction():
+    """Approaching a moving capsule with friction 10: the node takes the body's velocity
+    (p' + R' x_b - x) / dt at that point."""
+    v_out, bv, n, t = _capsule_case(10.0, -1.0, 0.2)
+    assert np.linalg.norm(bv) > 0.05
+    assert np.allclose(v_out, bv, atol=1e-9)
+
+
+def test_grid_op_coulomb_friction_on_moving_capsule():
+    """Friction 0.5, normal speed -1, tangential 0.8: the tangential relative speed
+    shrinks to 0.8 - 0.5 = 0.3 and the normal component is removed."""
+    v_out, bv, n, t = _capsule_case(0.5, -1.0, 0.8)
+    assert np.allclose(v_out, bv + 0.3 * t, atol=1e-9)
+    # separating from the moving capsule: relative velocity kept
+    v_out, bv, n, t = _capsule_case(0.5, 0.4, 0.8)
+    assert np.allclose(v_out, bv + 0.4 * n + 0.8 * t, atol=1e-9)
+
+
+@pytest.mark.parametrize("d,round_", [(0.002, 0.0), (0.003, 0.001), (0.004, 0.0)])
+def test_grid_op_influence_shell_beyond_capsule_cap(d, round_):
+    """Node on the capsule axis beyond the segment end y = h/2, at distance d from the
+    surface (the segment spans y in [-h/2, h/2]; h is its full length). Inside the shell
+    (exp(-(d - round) * 666) > 0.1) the contact is blended with the influence; outside it
+    the node is untouched."""
+    r, h, mu = 0.03, 0.1, 0.25
+    q, R = _rotq([0.5, 0.1, 1.0], 1.3)
+    x_b = np.array([0.0, h / 2 + r + d, 0.0])
+    node = (8, 7, 9)
+    p, q0, pn, qn = _placed(node, R, q, x_b)
+    n = R @ np.array([0.0, 1.0, 0.0])
+    t = np.cross(n, [1.0, 0, 0])
+    t /= np.linalg.norm(t)
+    v = -0.6 * n + 0.4 * t
+    cap = ([1, mu, 666, round_], [r, h, 0, 0], p, q0, pn, qn)
+    v_out, _, _, _ = _one_node(node, v, [cap])
+    infl = np.exp(-(d - round_) * 666)
+    if infl > 0.1:
+        vt = (0.4 - 0.6 * mu) * t
+        expect = v * (1 - infl) + vt * infl
+        assert 0.1 < infl < 1
+    else:
+        expect = v
+    assert np.allclose(v_out, expect, atol=1e-10)
+
+
+def test_grid_op_domain_walls_are_asymmetric():
+    """Low side: index < 3 stops outward motion; high side: index > dim - 3, i.e. only the
+    last two planes (integrator.cu:735-774)."""
+    d = GD[0]
+    cases = []
+    for i, vx, stop in [(1, -0.5, True), (2, -0.5, True), (3, -0.5, False), (1, 0.5, False),
+                        (d - 3, 0.5, False), (d - 2, 0.5, True), (d - 1, 0.5, True), (d - 2, -0.5, False)]:
+        cases.append(((i, 8, 8), np.array([vx, 0.25, -0.125]), (0, stop)))
+        cases.append(((8, 8, i), np.array([0.125, 0.25, vx]), (2, stop)))
+        if i > 3:  # the low y side is the ground (tested separately)
+            cases.append(((8, i, 8), np.array([0.125, vx, 0.25]), (1, stop)))
+    for node, v, (ax, stop) in cases:
+        v_out, _, _, _ = _one_node(node, v, [])
+        expect = v.copy()
+        if stop:
+            expect[ax] = 0.0
+        assert np.allclose(v_out, expect, atol=1e-12), (node, v, v_out)
+
+
+@pytest.mark.parametrize("gf", [0.0, 0.4, 100.0])
+def test_grid_op_ground_friction(gf):
+    """Ground (y index < 3, moving down): friction 0 only zeroes v_y; 0 < friction < 99
+    scales the tangential velocity by max(1 + gf v_y / sqrt(|v_t|^2 + 1e-8), 0); >= 99
+    sticks (v = 0). Moving up: untouched."""
+    for j in (0, 1, 2):
+        for vy, vx in ((-1.5, 1.0), (-3.0, 1.0), (0.5, 1.0)):
+            v = np.array([vx, vy, 0.0])
+            v_out, _, _, _ = _one_node((8, j, 8), v, [], gf=gf)
+            if vy > 0:
+                expect = v
+            elif gf == 0.0:
+                expect = np.array([vx, 0.0, 0.0])
+            elif gf >= 99:
+                expect = np.zeros(3)
+            else:
+                expect = np.array([vx * max(1 + gf * vy / np.sqrt(vx * vx + 1e-8), 0.0), 0.0, 0.0])
+            assert np.allclose(v_out, expect, atol=1e-12), (gf, j, vy, v_out)
+    v = np.array([1.0, -1.5, 0.0])
+    v_out, _, _, _ = _one_node((8, 3, 8), v, [], gf=gf)
+    assert np.allclose(v_out, v, atol=1e-12)
+    if 0 < gf < 99:
+        v_out, _, _, _ = _one_node((8, 1, 8), np.array([0.6, -1.5, 0.8]), [], gf=gf)
+        assert np.allclose(v_out, np.array([0.6, 0, 0.8]) * 0.4, atol=1e-8)
