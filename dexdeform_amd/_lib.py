@@ -90,18 +90,35 @@ assert STATE_DTYPE.itemsize == ctypes.sizeof(DdmState) == 48
 
 
 # Symbols declared in include/dexdeform_mpm.h
-REFERENCE_SYMBOLS = ["compute_svd", "compute_svd_grad", "p2g", "p2g_grad", "grid_op_v2", "grid_op_v2_grad", "g2p",
+RUNTIME_SYMBOLS = ["cuda_alloc", "cuda_free", "print_memory_info", "cuda_stream_create", "cuda_stream_destroy",
+                   "cuda_stream_sync", "cuda_upload", "cuda_download", "cuda_copy", "cuda_copy2d", "cuda_zero",
+                   "cuda_zero_async", "cuda_upload_async", "cuda_download_async", "cuda_copy_async"]
+REFERENCE_SYMBOLS = ["compute_grid_lower", "compute_svd", "compute_svd_grad", "p2g", "p2g_grad", "grid_op_v2", "grid_op_v2_grad", "g2p",
                      "g2p_grad", "compute_dist", "particle2mass"]
 FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_forward_timed",
                  "ddm_substep_backward_timed", "ddm_substep_forward_taped", "ddm_substep_backward_taped",
                  "ddm_substep_forward_taped_timed", "ddm_substep_backward_taped_timed", "ddm_batch_reset",
                  "ddm_version"]
-ALL_SYMBOLS = REFERENCE_SYMBOLS + FUSED_SYMBOLS
+ALL_SYMBOLS = RUNTIME_SYMBOLS + REFERENCE_SYMBOLS + FUSED_SYMBOLS
 
 _P = c_void_p
 _I = ctypes.c_int
 _F = c_float
+_S = ctypes.c_size_t
 _ARGTYPES = {
+    "cuda_free": [_P],
+    "cuda_stream_destroy": [_P],
+    "cuda_stream_sync": [_P],
+    "cuda_upload": [_P, _P, _S],
+    "cuda_download": [_P, _P, _S],
+    "cuda_copy": [_P, _P, _S],
+    "cuda_copy2d": [_P, _S, _P, _S, _S, _S],
+    "cuda_zero": [_P, _S],
+    "cuda_zero_async": [_P, _S, _P],
+    "cuda_upload_async": [_P, _P, _S, _P],
+    "cuda_download_async": [_P, _P, _S, _P],
+    "cuda_copy_async": [_P, _P, _S, _P],
+    "compute_grid_lower": [_P, _F, _F, _P, _I, _P],
     "compute_svd": [_P, _P, _P, _P, _P, _P, _F, _I, _P],
     "compute_svd_grad": [_P] * 11 + [_F, _I, _P],
     "p2g": [_P] * 11 + [_P, _F, _F, _F, _P, _P, _P, _I, _P],
@@ -145,6 +162,12 @@ def load(path: str = LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    lib.cuda_alloc.argtypes = [_S]
+    lib.cuda_alloc.restype = _P
+    lib.cuda_stream_create.argtypes = []
+    lib.cuda_stream_create.restype = _P
+    lib.print_memory_info.argtypes = []
+    lib.print_memory_info.restype = None
     _lib = lib
     return lib
 

@@ -25,6 +25,7 @@
 #ifndef DEXDEFORM_MPM_H
 #define DEXDEFORM_MPM_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
 
@@ -36,8 +37,50 @@ extern "C" {
 #define DDM_EINVAL 10001   /* bad argument (null pointer, size, dims not multiple of 4) */
 #define DDM_ETOOMANYBODIES 10002
 
+/* ---------------------------------------------------------------- group 0
+ * Device memory, copy and stream helpers under the names the reference's ctypes layer
+ * binds (mpm/types.py:103-128), so that its `array` class and MPMSimulator's streams run
+ * on this library unchanged. Each replaces the integrator.cu function of the same name
+ * (integrator.cu:1987-2072). Where the reference returns void these return 0 or the
+ * failing hipError_t; cuda_alloc and cuda_stream_create return NULL on failure. */
+
+/* replaces cuda_alloc, integrator.cu:1990-1995 */
+void *cuda_alloc(size_t size);
+/* replaces cuda_free, integrator.cu:2011 */
+int cuda_free(void *ptr);
+/* replaces print_memory_info, integrator.cu:1997-2009 (called by mpm/simulator.py:226) */
+void print_memory_info(void);
+/* replaces cuda_stream_create, integrator.cu:2013-2018 (default priority) */
+hipStream_t cuda_stream_create(void);
+/* replaces cuda_stream_destroy, integrator.cu:2020-2023 */
+int cuda_stream_destroy(hipStream_t stream);
+/* replaces cuda_stream_sync, integrator.cu:2069-2072 */
+int cuda_stream_sync(hipStream_t stream);
+/* replaces cuda_upload, integrator.cu:2025-2028 (synchronous host -> device) */
+int cuda_upload(void *device_ptr, void *host_ptr, size_t size);
+/* replaces cuda_download, integrator.cu:2030-2033 (synchronous device -> host) */
+int cuda_download(void *host_ptr, void *device_ptr, size_t size);
+/* replaces cuda_copy, integrator.cu:2035-2038 (device -> device) */
+int cuda_copy(void *dst, void *src, size_t size);
+/* replaces cuda_copy2d, integrator.cu:2040-2045 (device -> device, pitched) */
+int cuda_copy2d(void *dst, size_t dpitch, void *src, size_t spitch, size_t width, size_t height);
+/* replaces cuda_zero, integrator.cu:2047 */
+int cuda_zero(void *ptr, size_t size);
+/* replaces cuda_zero_async, integrator.cu:2048 */
+int cuda_zero_async(void *ptr, size_t size, hipStream_t stream);
+/* replaces cuda_upload_async, integrator.cu:2050-2055 */
+int cuda_upload_async(void *device_ptr, void *host_ptr, size_t size, hipStream_t stream);
+/* replaces cuda_download_async, integrator.cu:2057-2062 */
+int cuda_download_async(void *host_ptr, void *device_ptr, size_t size, hipStream_t stream);
+/* replaces cuda_copy_async, integrator.cu:2064-2067 */
+int cuda_copy_async(void *dst, void *src, size_t size, hipStream_t stream);
+
 /* ---------------------------------------------------------------- group 1
  * Reference-shaped launchers. n / dim = number of particles. */
+
+/* replaces compute_grid_lower, integrator.cu:70-82, 1619-1625: min-reduces
+ * floor((x - 10 dx) / dx) over the particles into the 3 ints at grid_lower (device) */
+int compute_grid_lower(float *particle_x, float dx, float inv_dx, int *grid_lower, int dim, hipStream_t stream);
 
 /* replaces compute_svd, integrator.cu:1627-1638 */
 int compute_svd(float *F, float *C, float *newF, float *U, float *V, float *sig, float dt, int dim,

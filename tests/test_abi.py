@@ -12,7 +12,8 @@ HEADER = os.path.join(ROOT, "include", "dexdeform_mpm.h")
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*int\s+([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*?|hipStream_t)\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt,
+                                 flags=re.M)))
 
 
 def test_header_declares_reference_launchers():
@@ -42,3 +43,26 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.DdmDet) == 10 * 8
     hdr = open(HEADER).read()
     assert "} ddm_scene;                   /* 176 bytes */" in hdr
+
+
+# Every symbol the reference's ctypes layer binds (mpm/types.py:103-290) or calls
+# (print_memory_info, mpm/simulator.py:226), except the renderer's (render, particle_sdf,
+# create_volume, destroy_volume: the renderer is not part of this library).
+REFERENCE_CTYPES_SYMBOLS = [
+    "cuda_alloc", "cuda_free", "cuda_upload", "cuda_upload_async", "cuda_download", "cuda_download_async",
+    "cuda_copy", "cuda_copy2d", "cuda_zero", "cuda_zero_async", "cuda_stream_create", "cuda_stream_destroy",
+    "cuda_stream_sync", "compute_grid_lower", "compute_svd", "compute_svd_grad", "p2g", "p2g_grad", "grid_op_v2",
+    "grid_op_v2_grad", "g2p", "g2p_grad", "compute_dist", "particle2mass", "print_memory_info"]
+
+
+def test_library_exports_every_symbol_the_reference_binds():
+    """The reference's mpm/types.py can load this library: each non-renderer symbol it
+    binds is exported (checked with a plain ctypes.CDLL, as the reference loads it) and
+    declared in the header."""
+    import ctypes
+    from dexdeform_amd import _lib
+    import torch  # noqa: F401  (the HIP runtime first, as the reference's process has it)
+    raw = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in REFERENCE_CTYPES_SYMBOLS if not hasattr(raw, s)]
+    assert not missing, missing
+    assert set(REFERENCE_CTYPES_SYMBOLS) <= set(header_functions())
