@@ -55,9 +55,18 @@ TRAFFIC_BYTES = {
 }
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
-# the taped rollout: its forward P2G also writes the temporaries; its adjoint restores the
-# grid snapshot (no recompute unless the snapshot overflowed)
-KERNEL_BYTES_TAPED = dict(KERNEL_BYTES, p2g_fwd=152 + 120, grid_fwd=0, restore=0)
+# the taped rollout: its forward P2G also writes the temporaries (120 B per particle) and
+# its adjoint restores the grid snapshot (no recompute unless the snapshot overflowed).
+# The tape is an implementation choice, not part of the path's algorithmic traffic: like
+# checkpoint recompute it is overhead, so the taped P2G is credited the same 152 B.
+KERNEL_BYTES_TAPED = dict(KERNEL_BYTES, restore=0)
+# VALU issue: each of a CU's 4 SIMDs issues one wave64 VALU instruction per 4 cycles, at
+# the 2.4 GHz peak engine clock on 256 CUs (FP64 instructions take 2 issue slots)
+VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
+# VALU wave-instructions per wave (64 particles) and the VALUBusy fraction per kernel,
+# from rocprofv3 SQ_INSTS_VALU / SQ_WAVES and SQ_ACTIVE_INST_VALU on the default workload
+VALU_INSTR_PER_WAVE = {"p2g_fwd": 5331, "g2p_fwd": 1265, "g2p_adj": 3607, "p2g_adj": 3476}
+VALU_BUSY = {"p2g_fwd": 0.70, "g2p_fwd": 0.89, "g2p_adj": 0.62, "p2g_adj": 0.72}
 FWD_NAMES_TAPED = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES_TAPED = ["restore", "g2p_adj", "grid_adj", "p2g_adj"]
 
@@ -80,7 +89,7 @@ def make_batched(n_scenes, seed, T):
     return scenes, poses
 
 
-def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False, deterministic=False):
+def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False, deterministic=False, gx=None):
     import torch
     from dexdeform_amd.mpm.engine import Engine
     from dexdeform_amd.mpm.rollout import Rollout
@@ -91,9 +100,10 @@ def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False, d
     if eng.n_bodies_total:
         ro.set_poses(torch.tensor(np.concatenate([p[0] for p in poses], 1), device=device),
                      torch.tensor(np.concatenate([p[1] for p in poses], 1), device=device))
-    rng = np.random.Generator(np.random.PCG64(11))
-    gx = torch.tensor(rng.normal(size=(eng.n_particles_total, 3)).astype(np.float32) / eng.n_particles_total,
-                      device=device)
+    if gx is None:  # loss seed dL/dx_T
+        rng = np.random.Generator(np.random.PCG64(11))
+        gx = torch.tensor(rng.normal(size=(eng.n_particles_total, 3)).astype(np.float32) / eng.n_particles_total,
+                          device=device)
     return eng, ro, gx
 
 
@@ -327,8 +337,10 @@ class KernelTimer:
 
 
 def cpu_baseline(budget_s=20.0):
-    """The oracle (OpenMP build) on the plain 10k-particle scene: forward + adjoint
-    substeps, timed on the host cores."""
+    """The oracle (OpenMP build, FP32) timed on the host cores: repeated forward +
+    adjoint passes of the plain 10k-particle scene (50 substeps each, as the plain
+    workload) for about two thirds of the budget, then forward substeps of the
+    50k-particle hand-contact scene for the rest."""
     from oracle import oracle as O
     from dexdeform_amd.mpm import workloads as W
     sc = W.plain_scene(n_particles=10000)
@@ -337,24 +349,83 @@ def cpu_baseline(budget_s=20.0):
                   mly=s.mu_lam_yield, tfsr=s.body_prim, args=s.body_args)
     osim = O.OracleSim(osc, np.float32, omp=True)
     n = sc.n
-    st = O.State(sc.x0, np.zeros((n, 3), np.float32), np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)),
-                 np.zeros((n, 3, 3), np.float32), sc.body_pos, sc.body_rot)
-    steps = 0
-    t0 = time.perf_counter()
-    states = [st]
-    while time.perf_counter() - t0 < budget_s / 2 and steps < 200:
-        states.append(osim.substep(states[-1], sc.body_pos, sc.body_rot))
-        steps += 1
+    st0 = O.State(sc.x0, np.zeros((n, 3), np.float32), np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)),
+                  np.zeros((n, 3, 3), np.float32), sc.body_pos, sc.body_rot)
+    T = 50
     g = O.Grads(np.ones((n, 3), np.float32) / n, np.zeros((n, 3), np.float32), np.zeros((n, 3, 3), np.float32),
                 np.zeros((n, 3, 3), np.float32), np.zeros((2, 3), np.float32), np.zeros((2, 4), np.float32))
-    gs = [O.Grads.zeros(n, 2, np.float32) for _ in range(steps + 1)]
-    gs[-1] = g
-    for t in range(steps - 1, -1, -1):
-        osim.substep_grad(states[t], states[t + 1], gs[t + 1], gs[t])
+    passes = 0
+    t0 = time.perf_counter()
+    while passes == 0 or time.perf_counter() - t0 < budget_s * 2 / 3:
+        states = [st0]
+        for _ in range(T):
+            states.append(osim.substep(states[-1], sc.body_pos, sc.body_rot))
+        gs = [O.Grads.zeros(n, 2, np.float32) for _ in range(T + 1)]
+        gs[-1] = g
+        for t in range(T - 1, -1, -1):
+            osim.substep_grad(states[t], states[t + 1], gs[t + 1], gs[t])
+        passes += 1
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-substeps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"oracle OpenMP fp32, plain scene 10k particles 64^3 2 capsules, {steps} substeps forward "
-                      f"+ {steps} adjoint substeps ({dt:.1f} s)"}
+    out = {"value": n * T * passes / dt, "unit": "particle-substeps/s", "cores": O.num_threads(), "kind": "port",
+           "sample": f"oracle OpenMP fp32 on {O.num_threads()} host threads: plain scene (10k particles, 64^3, "
+                     f"2 capsules), {passes} passes of {T} substeps forward + {T} adjoint substeps ({dt:.1f} s)"}
+    # forward only, hand-contact scene (50k particles, 22 capsule links)
+    hs, tree, q0 = W.hand_scene(50000)
+    hsp = hs.spec
+    hosc = O.Scene(grid_dim=tuple(hsp.grid_dim), dx=hsp.dx, dt=hsp.dt, gravity=tuple(hsp.gravity), mass=hsp.mass,
+                   vol=hsp.vol, mly=hsp.mu_lam_yield, tfsr=hsp.body_prim, args=hsp.body_args,
+                   ground_friction=float(hsp.ground_friction))
+    hsim = O.OracleSim(hosc, np.float32, omp=True)
+    nh = hs.n
+    st = O.State(hs.x0, np.zeros((nh, 3), np.float32), np.tile(np.eye(3, dtype=np.float32), (nh, 1, 1)),
+                 np.zeros((nh, 3, 3), np.float32), hs.body_pos, hs.body_rot)
+    k = 0
+    t1 = time.perf_counter()
+    while k == 0 or (time.perf_counter() - t1 < budget_s / 3 and k < 400):
+        st = hsim.substep(st, hs.body_pos, hs.body_rot)
+        k += 1
+    dh = time.perf_counter() - t1
+    out["forward_only_hand_50k"] = {"value": nh * k / dh, "unit": "particle-substeps/s (forward only)",
+                                    "sample": f"hand-contact scene, 50k particles, 22 links, {k} forward substeps "
+                                              f"({dh:.1f} s)"}
+    return out
+
+
+def _oracle_scene(O, spec):
+    return O.Scene(grid_dim=tuple(int(g) for g in spec.grid_dim), dx=float(np.float32(spec.dx)),
+                   dt=float(np.float32(spec.dt)), gravity=tuple(np.float32(spec.gravity)), mass=np.asarray(spec.mass),
+                   vol=np.asarray(spec.vol), mly=np.asarray(spec.mu_lam_yield), tfsr=np.asarray(spec.body_prim),
+                   args=np.asarray(spec.body_args), ground_friction=float(spec.ground_friction),
+                   ground_height=float(spec.ground_height))
+
+
+def oracle_check(scene, pose, gx_slice, x_final, g_x0):
+    """Part of the CPU-baseline leg: one scene of the benchmarked batch run by the oracle
+    (OpenMP build) for the same substeps, FP32 and FP64, with the adjoint from the same
+    loss seed dL/dx_T. Relative L2 errors of the GPU's final positions and initial-position
+    gradient (taken from the batched rollout) against the FP32 oracle, with the FP32
+    oracle's own distance from the FP64 oracle beside them (the FP32 rounding noise)."""
+    from oracle import oracle as O
+    T = len(pose[0]) - 1
+    n, nb = scene.n, scene.nb
+    rel = lambda a, b: float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))  # noqa: E731
+    res = {}
+    t0 = time.perf_counter()
+    for dtype in (np.float32, np.float64):
+        osim = O.OracleSim(_oracle_scene(O, scene.spec), dtype, omp=True)
+        st = O.State(scene.x0, np.zeros((n, 3)), np.tile(np.eye(3), (n, 1, 1)), np.zeros((n, 3, 3)),
+                     pose[0][0], pose[1][0]).astype(dtype)
+        states = osim.rollout(st, pose[0], pose[1])
+        g_last = O.Grads(np.ascontiguousarray(gx_slice, dtype), np.zeros((n, 3), dtype), np.zeros((n, 3, 3), dtype),
+                         np.zeros((n, 3, 3), dtype), np.zeros((nb, 3), dtype), np.zeros((nb, 4), dtype))
+        g0, _, _ = osim.rollout_grad(states, g_last)
+        res[dtype] = (states[-1].x.astype(np.float64), g0.x.astype(np.float64))
+    (x32, g32), (x64, g64) = res[np.float32], res[np.float64]
+    return {"scene_particles": n, "bodies": nb, "substeps": T,
+            "traj_rel_err": rel(x_final, x32), "grad_rel_err": rel(g_x0, g32),
+            "oracle_fp32_vs_fp64_traj": rel(x32, x64), "oracle_fp32_vs_fp64_grad": rel(g32, g64),
+            "gpu_vs_fp64_traj": rel(x_final, x64), "gpu_vs_fp64_grad": rel(g_x0, g64),
+            "seconds": time.perf_counter() - t0}
 
 
 def main():
@@ -436,6 +507,9 @@ def main():
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = n_total * T / ((t1 - t0) / args.steps)
+    # results of the timed rollout, compared below with each scene run alone and with the oracle
+    x_final_b = ro.final_state().x.detach().clone()
+    g_x0_b = ro.initial_grad().x.detach().clone()
     if tape:
         log(f"taped adjoint: {ro.tape_recomputed()} substeps of the last segment recomputed; "
             f"tape {ro.tape.nbytes() / 2**30:.1f} GiB")
@@ -499,14 +573,24 @@ def main():
     achieved = bytes_per[dom] / (kt[dom] * 1e-3) / 1e9 if kt[dom] > 0 else 0.0
     sum_ms = sum(kt.values())
     alg = 480.0 * n_total / (sum_ms * 1e-3) / 1e9
+    waves = eng.n_chunks * 4  # 256-thread workgroups, one chunk each
+    valu = None
+    if dom in VALU_INSTR_PER_WAVE and kt[dom] > 0:
+        v_ach = VALU_INSTR_PER_WAVE[dom] * waves / (kt[dom] * 1e-3)
+        valu = {"bound": "valu", "kernel": dom, "achieved": v_ach, "peak": VALU_PEAK_WAVE_INSTR_S,
+                "unit": "wave64 VALU instr/s", "frac": v_ach / VALU_PEAK_WAVE_INSTR_S,
+                "instr_per_wave": VALU_INSTR_PER_WAVE[dom], "waves_per_launch": waves,
+                "valu_busy_counter": VALU_BUSY.get(dom)}
 
     # ---------------- plain: the same scenes one at a time through the single-scene path,
     # eager and replayed from a per-scene HIP graph
-    plain_value = plain_eager_value = None
+    plain_value = plain_eager_value = batched_vs_alone = None
     if not args.no_plain:
         tot_e, tot_g, tot_ps = 0.0, 0.0, 0
+        traj_errs, grad_errs = [], []
         for i, (sc, po) in enumerate(zip(scenes, poses)):
-            e1, r1, g1 = build_rollout([sc], [po], T, dev, tape=tape)
+            sl = eng.scene_slice(i)
+            e1, r1, g1 = build_rollout([sc], [po], T, dev, tape=tape, gx=gx[sl].clone())
             run_step(r1, g1)
             torch.cuda.synchronize()
             a = time.perf_counter()
@@ -521,11 +605,20 @@ def main():
             torch.cuda.synchronize()
             tot_g += time.perf_counter() - a
             tot_ps += sc.n * T
+            traj_errs.append(float(torch.linalg.norm(r1.final_state().x - x_final_b[sl]) /
+                                   torch.linalg.norm(x_final_b[sl])))
+            grad_errs.append(float(torch.linalg.norm(r1.initial_grad().x - g_x0_b[sl]) /
+                                   torch.linalg.norm(g_x0_b[sl])))
             del e1, r1, g1
             if (i + 1) % 16 == 0:
                 log(f"plain {i + 1}/{len(scenes)} scenes done")
         plain_value = tot_ps / tot_g
         plain_eager_value = tot_ps / tot_e
+        batched_vs_alone = {"scenes": len(scenes), "traj_rel_err_max": max(traj_errs),
+                            "traj_rel_err_median": float(np.median(traj_errs)), "grad_rel_err_max": max(grad_errs),
+                            "grad_rel_err_median": float(np.median(grad_errs))}
+        log(f"batched vs alone: trajectory rel err max {max(traj_errs):.2e}, gradient rel err max "
+            f"{max(grad_errs):.2e}")
 
     named = None if args.no_named else named_rollouts(dev, max(1, min(args.steps, 3)))
     tutorial = None if args.no_named else trajectory_optimisation(dev)
@@ -539,6 +632,12 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(args.cpu_budget)
+        k = int(np.argmin([sc.n for sc in scenes]))
+        sl = eng.scene_slice(k)
+        cpu["oracle_check"] = oracle_check(scenes[k], poses[k], gx[sl].cpu().numpy(), x_final_b[sl].cpu().numpy(),
+                                           g_x0_b[sl].cpu().numpy())
+        log("oracle check (smallest scene): " + json.dumps(cpu["oracle_check"]))
+    chk = cpu["oracle_check"] if cpu else None
 
     out = {
         "metric": "particle-substeps/s fwd+bwd, batched vs plain at <=1e-4 err; p50/p99 rollout ms",
@@ -565,13 +664,23 @@ def main():
         "plain_value": plain_value,
         "plain_eager_value": plain_eager_value,
         "batched_over_plain": (value / plain_value) if plain_value else None,
+        # accuracy of the benchmarked rollout: one scene against the oracle (FP32; the FP32
+        # oracle's distance from its FP64 run is the rounding-noise reference), and every
+        # scene batched against the same scene run alone
+        "traj_rel_err": chk["traj_rel_err"] if chk else None,
+        "grad_rel_err": chk["grad_rel_err"] if chk else None,
+        "batched_vs_alone": batched_vs_alone,
         "kernel_ms_per_launch": kt,
+        "kernel_time_source": "hipEvent pairs recorded around each stage kernel on its launch stream, averaged "
+                              f"over the {T} launches of one timed rollout",
         "algorithmic_fwd_bwd_GBps": alg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
+                     "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_particle": kb[dom],
+                     "particles_per_launch": n_total, "kernel_ms": kt[dom],
                      "traffic": (TRAFFIC_BYTES[("taped_" if tape else "") + dom] * n_total
                                  if (args.scenes, args.seed) == (64, 7) and
                                  ("taped_" if tape else "") + dom in TRAFFIC_BYTES else None)},
+        "roofline_valu": valu,
         "single_scene_rollouts": named,
         "trajectory_optimisation": tutorial,
         "serve": serve,
