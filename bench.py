@@ -78,15 +78,7 @@ def log(*a):
 def make_batched(n_scenes, seed, T):
     from dexdeform_amd.mpm import workloads as W
     scenes = W.batched_scenes(n_scenes=n_scenes, seed=seed)
-    rng = np.random.Generator(np.random.PCG64(seed + 1))
-    poses = []
-    for sc in scenes:
-        vel = rng.uniform(-0.5, 0.5, size=(sc.nb, 3)).astype(np.float32)
-        t = np.arange(T + 1, dtype=np.float32)[:, None, None] * sc.spec.dt
-        pos = sc.body_pos[None] + t * vel[None]
-        rot = np.broadcast_to(sc.body_rot[None], (T + 1, sc.nb, 4))
-        poses.append((pos.astype(np.float32), np.ascontiguousarray(rot, np.float32)))
-    return scenes, poses
+    return scenes, W.constant_velocity_poses(scenes, T, seed + 1)
 
 
 def build_rollout(scenes, poses, T, device, checkpoint_every=None, tape=False, deterministic=False, gx=None):

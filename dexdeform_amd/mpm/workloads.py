@@ -174,6 +174,21 @@ def batched_scenes(n_scenes=64, seed=7, n_range=(5000, 60000), grid=64, dt=5e-5,
     return out
 
 
+def constant_velocity_poses(scenes, T: int, seed: int):
+    """Body poses [T+1, nb, 3] / [T+1, nb, 4] per scene for capsules translating with
+    constant velocities U(-0.5, 0.5) m/s (rotations fixed), drawn scene by scene from one
+    generator."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for sc in scenes:
+        vel = rng.uniform(-0.5, 0.5, size=(sc.nb, 3)).astype(np.float32)
+        t = np.arange(T + 1, dtype=np.float32)[:, None, None] * np.float32(sc.spec.dt)
+        pos = sc.body_pos[None] + t * vel[None]
+        rot = np.broadcast_to(sc.body_rot[None], (T + 1, sc.nb, 4))
+        out.append((pos.astype(np.float32), np.ascontiguousarray(rot, np.float32)))
+    return out
+
+
 def served_stream(rate: float, n_requests: int, seed=13, n_range=(5000, 100000), grids=(48, 64, 96),
                   horizon_range=(20, 200), checkpoint_every=10, dt=5e-5, ppc=6.0):
     """Rollout requests of a Poisson arrival stream (``rate`` requests/s): particle counts

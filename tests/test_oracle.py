@@ -583,3 +583,16 @@ def test_grid_op_ground_friction(gf):
     if 0 < gf < 99:
         v_out, _, _, _ = _one_node((8, 1, 8), np.array([0.6, -1.5, 0.8]), [], gf=gf)
         assert np.allclose(v_out, np.array([0.6, 0, 0.8]) * 0.4, atol=1e-8)
+
+
+def test_full_size_hand_fixture_adjoint_matches_its_finite_differences():
+    """The FP64 oracle adjoint stored for the full-size hand-contact configuration (50,000
+    particles, 200 substeps; tests/golden/make_fixtures.py) against FP64 central differences
+    of the oracle's forward pass stored beside it: within 1e-3 at eps 2.5e-5. At eps 1e-4
+    one coordinate's difference crosses the contact-influence threshold (influence > 0.1,
+    a jump of the reference's contact model) and the relative error of the 16 is 0.33."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "hand_c2_fp64.npz"), allow_pickle=False)
+    g = z["grad"][z["coords"][:, 0], z["coords"][:, 1]]
+    fd = z["fd_eps_2p5e5"]
+    assert np.linalg.norm(g - fd) / np.linalg.norm(fd) <= 1e-3
