@@ -250,6 +250,53 @@ def trajectory_optimisation(dev, iters=2):
             "particles": n, "bodies": nb}
 
 
+def score_benchmark(dev, n=100000, m=200000, reps=3):
+    """The shape score of tutorials/4_computing_score.ipynb cell 3 on the device: debiased
+    Sinkhorn divergence (p = 1, blur 1e-4) between 100,000 and 200,000 Gaussian points, value
+    and gradient w.r.t. the first cloud; and the Chamfer distance of the same clouds. Reports
+    seconds per call and cost-pair evaluations per second (each softmin over an n x m pair
+    set evaluates n m pairs; the Sinkhorn loop runs 4 softmins per epsilon value plus the
+    initial and final steps, the gradient 4 plan reductions)."""
+    import torch
+    from dexdeform_amd.mpm import losses as L
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn(n, 3, generator=g).to(dev).requires_grad_(True)
+    y = torch.randn(m, 3, generator=g).to(dev)
+    loss = L.SamplesLoss("sinkhorn", p=1, blur=1e-4)
+    n_eps = len(L.epsilon_schedule(1, L._diameter(x.detach(), y), 1e-4, 0.5))
+    sweep = 2 * n * m + n * n + m * m  # pairs of one epsilon's four softmins
+    pairs_fwd = sweep * (n_eps + 2)
+    pairs_bwd = 2 * n * m + 2 * n * n
+    s = loss(x, y)
+    s.backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x.grad = None
+        s = loss(x, y)
+        torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x.grad = None
+        s = loss(x, y)
+        s.backward()
+        torch.cuda.synchronize()
+    t_all = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c = L.chamfer_distance(x, y)
+        torch.cuda.synchronize()
+    t_ch = (time.perf_counter() - t0) / reps
+    out = {"points": [n, m], "p": 1, "blur": 1e-4, "epsilon_steps": n_eps, "sinkhorn": float(s.item()),
+           "seconds_value": t_fwd, "seconds_value_and_grad": t_all,
+           "pairs_per_s_value": pairs_fwd / t_fwd, "pairs_per_s_value_and_grad": (pairs_fwd + pairs_bwd) / t_all,
+           "chamfer": float(c.item()), "chamfer_seconds": t_ch, "chamfer_pairs_per_s": 2 * n * m / t_ch}
+    log(f"score: sinkhorn {n}x{m} {t_fwd:.3f} s value, {t_all:.3f} s with gradient "
+        f"({out['pairs_per_s_value']:.3e} pairs/s); chamfer {t_ch * 1e3:.1f} ms")
+    return out
+
+
 def run_step(ro, gx):
     ro.forward()
     ro.backward(x_grad=gx)
@@ -432,6 +479,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-score", action="store_true", help="skip the shape-score (Sinkhorn, Chamfer) leg")
     ap.add_argument("--no-tape", action="store_true",
                     help="adjoint recomputes P2G and the grid update (as the reference does) instead of "
                          "reading the forward pass's tape")
@@ -620,6 +668,8 @@ def main():
         serve = serve_benchmark(dev, [float(r) for r in args.serve_rates.split(",")], args.serve_seconds, args.seed,
                                 not args.serve_eager)
 
+    score = None if args.no_score else score_benchmark(dev)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -676,6 +726,7 @@ def main():
         "single_scene_rollouts": named,
         "trajectory_optimisation": tutorial,
         "serve": serve,
+        "score": score,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

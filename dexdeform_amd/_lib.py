@@ -99,7 +99,8 @@ FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_for
                  "ddm_substep_backward_timed", "ddm_substep_forward_taped", "ddm_substep_backward_taped",
                  "ddm_substep_forward_taped_timed", "ddm_substep_backward_taped_timed", "ddm_batch_reset",
                  "ddm_version"]
-ALL_SYMBOLS = RUNTIME_SYMBOLS + REFERENCE_SYMBOLS + FUSED_SYMBOLS
+SCORE_SYMBOLS = ["ddm_pairwise_workspace_bytes", "ddm_softmin", "ddm_plan_grad", "ddm_nearest"]
+ALL_SYMBOLS = RUNTIME_SYMBOLS + REFERENCE_SYMBOLS + FUSED_SYMBOLS + SCORE_SYMBOLS
 
 _P = c_void_p
 _I = ctypes.c_int
@@ -140,6 +141,9 @@ _ARGTYPES = {
     "ddm_substep_backward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
     "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],
     "ddm_version": [],
+    "ddm_softmin": [_P, _I, _P, _P, _I, _F, _I, _I, _P, _S, _P, _P],
+    "ddm_plan_grad": [_P, _P, _I, _P, _P, _I, _F, _I, _I, _P, _S, _P, _P],
+    "ddm_nearest": [_P, _I, _P, _I, _I, _P, _S, _P, _P, _P],
 }
 
 _lib = None
@@ -168,6 +172,8 @@ def load(path: str = LIB_PATH):
     lib.cuda_stream_create.restype = _P
     lib.print_memory_info.argtypes = []
     lib.print_memory_info.restype = None
+    lib.ddm_pairwise_workspace_bytes.argtypes = [_I, _I]
+    lib.ddm_pairwise_workspace_bytes.restype = _S
     _lib = lib
     return lib
 

@@ -28,15 +28,22 @@ Requests of different particle count, grid size and horizon share the GPU:
 * gradients: the adjoint launches write (not add to) the state-t gradients
   (DDM_GRAD_OVERWRITE), so no clearing kernels run between ticks.
 
+* persistent loop: ``start()`` runs the tick loop in a background thread; requests
+  submitted with ``submit_async`` from any thread while it runs join at the next tick and
+  each returns a future resolved with its ``Result`` (``poll()`` runs one iteration for a
+  caller that drives the loop itself; ``run`` replays a fixed arrival trace).
+
 Latency of a request = completion (an event recorded after its last operation) minus
 arrival; throughput = particle-substeps (forward + adjoint) per second of wall time.
 """
 from __future__ import annotations
 
 import ctypes
-
 import heapq
+import queue
+import threading
 import time
+from concurrent.futures import Future
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -549,9 +556,38 @@ class RolloutServer:
         self.ticks = 0
         self.particle_substeps = 0
         self.replays = 0  # ticks that replayed a captured graph
+        # persistent loop: submissions from other threads, futures of unfinished requests
+        self._inbox: "queue.SimpleQueue" = queue.SimpleQueue()
+        self._futures: Dict[int, Future] = {}
+        self._thread: Optional[threading.Thread] = None
+        self._stop = threading.Event()
+        self._outstanding = set()  # futures not yet resolved
+        self._outstanding_lock = threading.Lock()
 
     # ------------------------------------------------------------------ admission
     def submit(self, req: Request):
+        """Queues a request of a trace (arrival relative to the serving loop's start);
+        call from the thread that runs the loop (see submit_async for other threads)."""
+        self._validate(req)
+        self.waiting.append(req)
+
+    def submit_async(self, req: Request) -> Future:
+        """Queues a request from any thread while the loop runs (``start``/``poll``); it
+        arrives now and joins at the next tick. Returns a future resolved with its Result
+        (or with the exception that stopped the loop). Invalid requests raise here."""
+        self._validate(req)
+        fut: Future = Future()
+        with self._outstanding_lock:
+            self._outstanding.add(fut)
+        fut.add_done_callback(self._forget)
+        self._inbox.put((req, fut))
+        return fut
+
+    def _forget(self, fut):
+        with self._outstanding_lock:
+            self._outstanding.discard(fut)
+
+    def _validate(self, req: Request):
         if req.T < 1:
             raise ValueError(f"request {req.rid}: the horizon must be at least one substep")
         if req.checkpoint_every is not None and (req.checkpoint_every < 1 or req.T % req.checkpoint_every):
@@ -565,7 +601,6 @@ class RolloutServer:
         if (gd[0] // 4) * (gd[1] // 4) * (gd[2] // 4) > self.arena.n:
             raise ValueError(f"request {req.rid} grid exceeds the arena")
         req.spec.validate()
-        self.waiting.append(req)
 
     def interval(self, req: Request) -> int:
         """Checkpoint interval the request would run with if admitted now (see
@@ -738,9 +773,12 @@ class RolloutServer:
                 x_final, x0_grad = a.x_final, g0.x.clone()
                 if a.inv is not None:  # back to the request's particle order
                     x_final, x0_grad = x_final[a.inv], x0_grad[a.inv]
-                self.results[a.req.rid] = Result(a.req.rid, float(a.loss_host.item()), x_final, a.pos_grad,
-                                                 a.rot_grad, x0_grad, a.req.arrival, a.admitted, fin,
-                                                 bool(a.finite_host.item()))
+                res = Result(a.req.rid, float(a.loss_host.item()), x_final, a.pos_grad, a.rot_grad, x0_grad,
+                             a.req.arrival, a.admitted, fin, bool(a.finite_host.item()))
+                self.results[a.req.rid] = res
+                fut = self._futures.pop(a.req.rid, None)
+                if fut is not None:
+                    fut.set_result(res)
                 self.used_bytes -= a.bytes
                 self.arena.release(a.blk_begin, a.n_blocks)
             else:
@@ -762,6 +800,88 @@ class RolloutServer:
                 raise TimeoutError("rollout server did not drain in time")
         torch.cuda.synchronize()
         return self.results
+
+
+    # ------------------------------------------------------------------ persistent loop
+    def _drain_inbox(self, first=None):
+        now = self._now()
+        while True:
+            if first is not None:
+                (req, fut), first = first, None
+            else:
+                try:
+                    req, fut = self._inbox.get_nowait()
+                except queue.Empty:
+                    return
+            if req.rid in self._futures or any(w.rid == req.rid for w in self.waiting) or \
+                    any(a.req.rid == req.rid for a in self.active):
+                fut.set_exception(ValueError(f"request id {req.rid} is already being served"))
+                continue
+            req.arrival = now
+            self.waiting.append(req)
+            self._futures[req.rid] = fut
+
+    def poll(self) -> bool:
+        """One iteration of the serving loop: takes submitted requests, admits, launches one
+        tick and collects finished requests (resolving their futures). Returns whether any
+        request is still waiting or running."""
+        if self.t0 is None:
+            self.t0 = time.perf_counter()
+        self._drain_inbox()
+        if not self.tick() and self.active and all(a.done_event is not None for a in self.active):
+            self._collect(block=True)
+        return bool(self.waiting or self.active)
+
+    def serve_forever(self, idle_wait: float = 1e-3):
+        """Runs ``poll`` until ``stop`` is called; sleeps on the inbox while idle. An
+        exception fails every unresolved future and ends the loop."""
+        try:
+            while not self._stop.is_set():
+                if not self.poll():
+                    try:
+                        item = self._inbox.get(timeout=idle_wait)
+                    except queue.Empty:
+                        continue
+                    self._drain_inbox(first=item)
+        except BaseException as e:  # noqa: BLE001 (handed to the waiting callers)
+            self._fail_all(e)
+            raise
+
+    def _fail_all(self, e):
+        for fut in list(self._futures.values()):
+            if not fut.done():
+                fut.set_exception(e)
+        self._futures.clear()
+        while True:
+            try:
+                _, fut = self._inbox.get_nowait()
+            except queue.Empty:
+                break
+            fut.set_exception(e)
+
+    def start(self) -> "RolloutServer":
+        """Starts the serving loop in a background thread (see submit_async)."""
+        if self._thread is not None and self._thread.is_alive():
+            raise RuntimeError("the serving loop is already running")
+        self._stop.clear()
+        self._thread = threading.Thread(target=self.serve_forever, name="rollout-server", daemon=True)
+        self._thread.start()
+        return self
+
+    def stop(self, drain: bool = True, timeout: float = 600.0):
+        """Stops the background loop, after the submitted requests have finished if drain."""
+        if self._thread is None:
+            return
+        if drain:
+            t_end = time.perf_counter() + timeout
+            while self._thread.is_alive() and self._outstanding:
+                if time.perf_counter() > t_end:
+                    raise TimeoutError("rollout server did not drain in time")
+                time.sleep(1e-3)
+        self._stop.set()
+        self._thread.join()
+        self._thread = None
+        torch.cuda.synchronize(self.device)
 
 
 def latency_stats(results: Dict[int, Result]):

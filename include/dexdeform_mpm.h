@@ -19,8 +19,9 @@
  *     (mpm/simulator.py:561-571) and ddm_substep_backward replaces
  *     MPMSimulator.substep_grad (mpm/simulator.py:573-585), for every scene at once.
  *
- * All work is enqueued on the given stream; no function synchronises the device or
- * allocates device memory, so the sequences can be captured in a HIP graph.
+ * Groups 1 and 2 enqueue all work on the given stream and neither synchronise the device
+ * nor allocate device memory, so their sequences can be captured in a HIP graph (group 0
+ * allocates, copies and synchronises, as the reference's helpers do).
  */
 #ifndef DEXDEFORM_MPM_H
 #define DEXDEFORM_MPM_H
@@ -286,6 +287,27 @@ int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, co
 int ddm_substep_backward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                      const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
                                      hipEvent_t *events);
+
+/* ---------------------------------------------------------------- group 3
+ * Point-cloud scores between simulated particles and a target shape, computed without
+ * forming the n x m cost matrix. No counterpart in the reference's library: the reference
+ * scores shapes with the geomloss package (SamplesLoss("sinkhorn", p=1, blur=1e-4),
+ * tutorials/4_computing_score.ipynb cells 1-7), whose Sinkhorn iterations reduce over all
+ * pairs with these two operations; the Chamfer distance uses the nearest-neighbour one.
+ * Points are float [n][3]; C(x, y) = |x - y| (p = 1) or |x - y|^2 / 2 (p = 2). The j range
+ * is split into `segments` pieces (more workgroups for small n); `workspace` holds
+ * ddm_pairwise_workspace_bytes(n, segments) bytes of per-segment partial results, combined
+ * in a fixed order (results are deterministic). */
+size_t ddm_pairwise_workspace_bytes(int n, int segments);
+/* out[i] = -eps * log(sum_j exp(h[j] - C(x_i, y_j) / eps)) */
+int ddm_softmin(const float *x, int n, const float *y, const float *h, int m, float eps, int p, int segments,
+                void *workspace, size_t workspace_bytes, float *out, hipStream_t stream);
+/* grad[i] = sum_j exp(a[i] + b[j] - C(x_i, y_j) / eps) * dC/dx(x_i, y_j)  (float [n][3]) */
+int ddm_plan_grad(const float *x, const float *a, int n, const float *y, const float *b, int m, float eps, int p,
+                  int segments, void *workspace, size_t workspace_bytes, float *grad, hipStream_t stream);
+/* dist2[i] = min_j |x_i - y_j|^2, index[i] = its j (the smallest on ties) */
+int ddm_nearest(const float *x, int n, const float *y, int m, int segments, void *workspace, size_t workspace_bytes,
+                float *dist2, int32_t *index, hipStream_t stream);
 
 /* Zeroes the batch's grid arenas and block flags (after an error or on setup). */
 int ddm_batch_reset(const ddm_batch *b, hipStream_t stream);

@@ -12,7 +12,7 @@ HEADER = os.path.join(ROOT, "include", "dexdeform_mpm.h")
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*?|hipStream_t)\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt,
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|void\s*\*?|hipStream_t)\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt,
                                  flags=re.M)))
 
 

@@ -1,0 +1,149 @@
+"""Shape scores on the GPU (libdexdeform_mpm's pair reductions) against dense FP64 numpy
+and the CPU oracle (oracle/sinkhorn.py).
+
+* ddm_softmin / ddm_plan_grad / ddm_nearest on clouds whose sizes are not multiples of the
+  tile or the segment length, with one, several and the maximum number of j segments:
+  potentials within 2e-6 absolute (values O(1); FP32 cost and log-sum-exp), plan gradients
+  within 1e-5 relative, nearest indices exact;
+* SamplesLoss("sinkhorn", p=1, blur=1e-4) -- the tutorial's score -- and p = 2, blur 0.05
+  on 1,500 x 2,000 points: loss within 1e-5 relative of the oracle, gradient within 1e-4;
+* Chamfer distance and gradient: 1e-6;
+* at 100,000 x 200,000 points (the tutorial's sizes), where the oracle cannot run: the
+  divergence of a cloud with itself is 0 and of a translated copy is the translation length.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sinkhorn as SK
+from tests.helpers import rel_err
+from dexdeform_amd import _lib
+from dexdeform_amd.mpm import losses as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def _t(a):
+    return torch.tensor(np.ascontiguousarray(a, np.float32), device=_dev())
+
+
+def _softmin_ref(x, y, h, eps, p):
+    return SK.softmin(eps, SK.cost(x.astype(np.float64), y.astype(np.float64), p), h.astype(np.float64))
+
+
+@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("segments", [1, 7, 777])
+def test_pair_kernels_against_dense_reductions(p, segments):
+    rng = np.random.default_rng(10 + segments)
+    n, m = 1001, 777
+    x = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    y = rng.uniform(0, 1, (m, 3)).astype(np.float32)
+    y[5] = x[3]  # a coincident pair
+    lib = _lib.load()
+    ws = torch.empty(int(lib.ddm_pairwise_workspace_bytes(n, segments)), dtype=torch.uint8, device=_dev())
+    xd, yd = _t(x), _t(y)
+    print(f"pair kernels: n={n} m={m} p={p} segments={segments}", flush=True)
+    for eps in (1e-4, 1e-2, 1.0):
+        h = (rng.normal(size=m) * 3).astype(np.float32)
+        out = torch.empty(n, device=_dev())
+        _lib.check(lib.ddm_softmin(xd.data_ptr(), n, yd.data_ptr(), _t(h).data_ptr(), m, eps, p, segments,
+                                   ws.data_ptr(), ws.numel(), out.data_ptr(), _lib.stream_ptr()))
+        ref = _softmin_ref(x, y, h, eps, p)
+        err = np.abs(out.cpu().numpy() - ref).max()
+        print(f"  softmin eps {eps:g}: max abs err {err:.2e}")
+        assert err <= 2e-6 * max(1.0, np.abs(ref).max())
+        a = (rng.normal(size=n) - 4).astype(np.float32)
+        b = (rng.normal(size=m) - 4).astype(np.float32)
+        C = SK.cost(x.astype(np.float64), y.astype(np.float64), p)
+        w = np.exp(a[:, None].astype(np.float64) + b[None, :] - C / eps)
+        diff = x[:, None, :].astype(np.float64) - y[None, :, :]
+        if p == 1:
+            r = np.sqrt((diff ** 2).sum(-1, keepdims=True))
+            diff = np.where(r > 0, diff / np.where(r > 0, r, 1), 0)
+        gref = (w[..., None] * diff).sum(1)
+        g = torch.empty(n, 3, device=_dev())
+        _lib.check(lib.ddm_plan_grad(xd.data_ptr(), _t(a).data_ptr(), n, yd.data_ptr(), _t(b).data_ptr(), m, eps, p,
+                                     segments, ws.data_ptr(), ws.numel(), g.data_ptr(), _lib.stream_ptr()))
+        e = rel_err(g.cpu().numpy(), gref)
+        print(f"  plan_grad eps {eps:g}: rel err {e:.2e}")
+        assert e <= 1e-5 or np.linalg.norm(gref) < 1e-30
+    d2 = torch.empty(n, device=_dev())
+    idx = torch.empty(n, dtype=torch.int32, device=_dev())
+    _lib.check(lib.ddm_nearest(xd.data_ptr(), n, yd.data_ptr(), m, segments, ws.data_ptr(), ws.numel(), d2.data_ptr(),
+                               idx.data_ptr(), _lib.stream_ptr()))
+    D = ((x[:, None, :].astype(np.float64) - y[None, :, :]) ** 2).sum(-1)
+    assert np.array_equal(idx.cpu().numpy(), D.argmin(1))
+    np.testing.assert_allclose(d2.cpu().numpy(), D.min(1), rtol=1e-5, atol=1e-12)
+    assert d2[3].item() == 0.0 and idx[3].item() == 5
+
+
+def test_pair_kernels_reject_bad_arguments():
+    lib = _lib.load()
+    ws = torch.empty(1 << 16, dtype=torch.uint8, device=_dev())
+    x = _t(np.zeros((10, 3)))
+    out = torch.empty(10, device=_dev())
+    h = _t(np.zeros(10))
+    s = _lib.stream_ptr()
+    assert lib.ddm_softmin(x.data_ptr(), 10, x.data_ptr(), h.data_ptr(), 10, 0.0, 1, 1, ws.data_ptr(), ws.numel(),
+                           out.data_ptr(), s) == _lib.DDM_EINVAL  # eps <= 0
+    assert lib.ddm_softmin(x.data_ptr(), 10, x.data_ptr(), h.data_ptr(), 10, 1.0, 3, 1, ws.data_ptr(), ws.numel(),
+                           out.data_ptr(), s) == _lib.DDM_EINVAL  # p not 1 or 2
+    assert lib.ddm_softmin(x.data_ptr(), 10, x.data_ptr(), h.data_ptr(), 10, 1.0, 1, 11, ws.data_ptr(), ws.numel(),
+                           out.data_ptr(), s) == _lib.DDM_EINVAL  # more segments than points
+    assert lib.ddm_softmin(x.data_ptr(), 10, x.data_ptr(), h.data_ptr(), 10, 1.0, 1, 2, ws.data_ptr(), 8,
+                           out.data_ptr(), s) == _lib.DDM_EINVAL  # workspace too small
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("p,blur", [(1, 1e-4), (2, 0.05)])
+def test_sinkhorn_divergence_matches_oracle(p, blur):
+    rng = np.random.default_rng(20 + p)
+    x = rng.uniform(0.3, 0.6, (1500, 3))
+    y = rng.uniform(0.35, 0.7, (2000, 3))
+    xd = _t(x).requires_grad_(True)
+    S = L.SamplesLoss("sinkhorn", p=p, blur=blur)(xd, _t(y))
+    S.backward()
+    S_ref, g_ref, _ = SK.sinkhorn(x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64),
+                                  p=p, blur=blur)
+    e_s = abs(S.item() - S_ref) / abs(S_ref)
+    e_g = rel_err(xd.grad.cpu().numpy(), g_ref)
+    print(f"sinkhorn p={p} blur={blur}: S {S.item():.8e} (oracle {S_ref:.8e}) rel err {e_s:.2e}; grad rel err {e_g:.2e}")
+    assert e_s <= 1e-5 and e_g <= 1e-4
+
+
+def test_chamfer_matches_oracle():
+    rng = np.random.default_rng(30)
+    x = rng.uniform(0, 1, (3000, 3))
+    y = rng.uniform(0, 1, (2500, 3))
+    xd = _t(x).requires_grad_(True)
+    c = L.chamfer_distance(xd, _t(y))
+    c.backward()
+    v, g = SK.chamfer(x.astype(np.float32), y.astype(np.float32))
+    print(f"chamfer {c.item():.8e} (oracle {v:.8e}); grad rel err {rel_err(xd.grad.cpu().numpy(), g):.2e}")
+    assert abs(c.item() - v) <= 1e-6 * v
+    assert rel_err(xd.grad.cpu().numpy(), g) <= 1e-6
+
+
+def test_tutorial_sized_score_properties():
+    """100,000 x 200,000 points, p = 1, blur 1e-4 (tutorials/4_computing_score.ipynb cell 3)."""
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn(100000, 3, generator=g).to(_dev())
+    loss = L.SamplesLoss("sinkhorn", p=1, blur=1e-4)
+    s0 = loss(x, x.clone()).item()
+    t = torch.tensor([0.3, -0.2, 0.1], device=_dev())
+    st = loss(x, x + t).item()
+    print(f"S(x, x) = {s0:.3e}; S(x, x + t) = {st:.6f} vs |t| = {t.norm().item():.6f}", flush=True)
+    assert abs(s0) <= 1e-5
+    assert abs(st - t.norm().item()) <= 1e-3 * t.norm().item()
+    y = torch.randn(200000, 3, generator=g).to(_dev())
+    xg = x.clone().requires_grad_(True)
+    s = loss(xg, y)
+    s.backward()
+    assert torch.isfinite(s) and torch.isfinite(xg.grad).all() and s.item() > 0

@@ -323,3 +323,45 @@ def test_taped_adjoint_matches_recompute(K):
     e = [rel_err(pg_mx, pg_ref), rel_err(g0_mx, g0_ref)]
     print(f"mixed restore/recompute (K={K}): pose grad {e[0]:.2e} x0 grad {e[1]:.2e}")
     assert e[0] <= 1e-4 and e[1] <= 1e-4
+
+
+def test_persistent_loop_serves_requests_submitted_while_it_runs():
+    """The serving loop runs in a background thread; requests are submitted from this
+    thread one by one while earlier ones are on the GPU, each future resolves with the
+    result the request gets alone, and the requests overlap on the device."""
+    import time as _time
+    reqs = _requests()
+    srv = RolloutServer(device=_dev(), arena_blocks=16384, memory_budget=8 << 30).start()
+    try:
+        futs = []
+        for r in reqs:
+            futs.append(srv.submit_async(r))
+            _time.sleep(0.004)
+        dup = srv.submit_async(copy.copy(reqs[-1]))  # same id while the first is being served
+        with pytest.raises(ValueError):
+            srv.submit_async(Request(99, reqs[0].spec, reqs[0].x0, reqs[0].pose_pos[:1], reqs[0].pose_rot[:1],
+                                     reqs[0].target))  # horizon 0: rejected in the caller's thread
+        results = {r.rid: f.result(timeout=300) for r, f in zip(reqs, futs)}
+        with pytest.raises(ValueError):
+            dup.result(timeout=60)
+    finally:
+        srv.stop()
+    events = sorted([(r.admitted, 1) for r in results.values()] + [(r.finished, -1) for r in results.values()])
+    live, most = 0, 0
+    for _, d in events:
+        live += d
+        most = max(most, live)
+    print(f"persistent loop: at most {most} requests in flight, {srv.ticks} ticks, "
+          f"latencies {[round(r.latency * 1e3, 1) for r in results.values()]} ms")
+    assert most >= 2, "requests submitted while others ran did not overlap"
+    for r in reqs:
+        fresh = Request(r.rid, r.spec, r.x0, r.pose_pos, r.pose_rot, r.target, arrival=0.0,
+                        checkpoint_every=r.checkpoint_every)
+        alone = RolloutServer(device=_dev(), arena_blocks=8192, memory_budget=8 << 30).run([fresh])[r.rid]
+        b = results[r.rid]
+        el = abs(alone.loss - b.loss) / abs(alone.loss)
+        ex = rel_err(b.x_final.cpu().numpy(), alone.x_final.cpu().numpy())
+        eg = rel_err(b.pos_grad.cpu().numpy(), alone.pos_grad.cpu().numpy())
+        e0 = rel_err(b.x0_grad.cpu().numpy(), alone.x0_grad.cpu().numpy())
+        print(f"request {r.rid}: loop vs alone loss {el:.2e} x {ex:.2e} pose grad {eg:.2e} x0 grad {e0:.2e}")
+        assert el <= 1e-4 and ex <= 1e-4 and eg <= 1e-3 and e0 <= 1e-3

@@ -1,0 +1,213 @@
+"""The batched and served configurations at workload size on the GPU.
+
+Batched uneven scenes: the 64-scene batch (particles U[5k, 60k], 64^3 grids, yield stress
+U[20, 100], 2-22 capsules with constant velocities, 100 substeps forward + backward, taped
+adjoint, replayed from a HIP graph) against sampled scenes run alone through the untaped,
+eager single-scene path; and a batch of three small scenes on 32^3, 48^3 and 64^3 grids
+against the FP32 oracle directly.
+
+Served rollouts: a Poisson stream (50 requests/s) of 24 requests from the served-rollout
+distribution (particles U[5k, 100k], grids 48^3 / 64^3 / 96^3, horizons U[20, 200] with
+checkpoints every 10) plus one small request, through the continuous-batching server;
+sampled requests against their stand-alone results, and the small one against the oracle.
+
+Tolerances: 1e-4 relative (L2) for final positions and losses, 1e-3 for gradients; FP32
+atomic accumulation order is the difference between the runs. Body-pose gradients are
+compared per substep between GPU runs, and against the oracle through what an action sees:
+the gradient of each capsule's constant velocity (sum over substeps of t dt dL/dpos_t)
+within 1e-3, and the rotation gradient summed over substeps within 1e-2: each substep's
+rotation gradient carries terms of order |dL/dv| / dt that cancel between consecutive
+substeps, so FP32 summation order alone moves the sum by a few 1e-3 (the FP32 oracle is
+4.0e-3 from the FP64 oracle on the 32^3 scene below; the GPU 2.1e-3 from the FP32 oracle).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.helpers import initial_state, oracle_scene, rel_err
+from dexdeform_amd.mpm import workloads as W
+from dexdeform_amd.mpm.engine import Engine
+from dexdeform_amd.mpm.rollout import Rollout
+from dexdeform_amd.mpm.serve import Request, RolloutServer, latency_stats
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def _seed_grad(n_total, seed=11):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return (rng.normal(size=(n_total, 3)) / n_total).astype(np.float32)
+
+
+def _batched(scenes, poses, T, gx, graph=True):
+    """Taped rollout of all scenes in one launch per stage (graph-replayed)."""
+    dev = _dev()
+    eng = Engine([s.spec for s in scenes], device=dev)
+    ro = Rollout(eng, T, tape=True)
+    ro.set_initial(torch.tensor(np.concatenate([s.x0 for s in scenes]), device=dev))
+    ro.set_poses(torch.tensor(np.concatenate([p[0] for p in poses], 1), device=dev),
+                 torch.tensor(np.concatenate([p[1] for p in poses], 1), device=dev))
+    g = torch.tensor(gx, device=dev)
+    if graph:
+        ro.capture(g)
+        ro.replay()
+    else:
+        ro.forward()
+        ro.backward(x_grad=g)
+    torch.cuda.synchronize()
+    return (eng, ro.final_state().x.cpu().numpy(), ro.initial_grad().x.cpu().numpy(), ro.pos_grad.cpu().numpy(),
+            ro.rot_grad.cpu().numpy(), ro.tape_recomputed())
+
+
+def _alone(sc, pose, T, gx):
+    """The scene alone through the untaped, eager single-scene path."""
+    dev = _dev()
+    eng = Engine([sc.spec], device=dev)
+    ro = Rollout(eng, T)
+    ro.set_initial(torch.tensor(sc.x0, device=dev))
+    ro.set_poses(torch.tensor(pose[0], device=dev), torch.tensor(pose[1], device=dev))
+    ro.forward()
+    ro.backward(x_grad=torch.tensor(gx, device=dev))
+    torch.cuda.synchronize()
+    return (ro.final_state().x.cpu().numpy(), ro.initial_grad().x.cpu().numpy(), ro.pos_grad.cpu().numpy(),
+            ro.rot_grad.cpu().numpy())
+
+
+def _velocity_grad(pos_grad, dt):
+    """Gradient w.r.t. constant body velocities: pos_t = pos_0 + t dt vel."""
+    t = np.arange(len(pos_grad), dtype=np.float64)[:, None, None] * dt
+    return (pos_grad.astype(np.float64) * t).sum(0)
+
+
+def test_batched_64_uneven_scenes_match_each_scene_alone():
+    T = 100
+    scenes = W.batched_scenes(n_scenes=64, seed=7)
+    poses = W.constant_velocity_poses(scenes, T, seed=8)
+    n_tot = sum(s.n for s in scenes)
+    print(f"batched: 64 scenes, {n_tot} particles, T={T}", flush=True)
+    gx = _seed_grad(n_tot)
+    eng, xT, g0, pg, rg, recomputed = _batched(scenes, poses, T, gx)
+    assert np.isfinite(xT).all() and np.isfinite(g0).all() and np.isfinite(pg).all()
+    counts = [s.n for s in scenes]
+    rng = np.random.default_rng(3)
+    sample = sorted({int(np.argmin(counts)), int(np.argmax(counts)), *rng.choice(64, 2, replace=False).tolist()})
+    for i in sample:
+        sl, bs = eng.scene_slice(i), eng.body_slice(i)
+        x1, g01, pg1, rg1 = _alone(scenes[i], poses[i], T, gx[sl])
+        e = dict(x=rel_err(xT[sl], x1), x0_grad=rel_err(g0[sl], g01), pos_grad=rel_err(pg[:, bs], pg1),
+                 rot_grad=rel_err(rg[:, bs], rg1))
+        print(f"scene {i} ({scenes[i].n} particles, {scenes[i].nb} capsules): batched vs alone "
+              + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
+        assert e["x"] <= 1e-4, e
+        assert max(e["x0_grad"], e["pos_grad"], e["rot_grad"]) <= 1e-3, e
+    print(f"(scene, substep) pairs recomputed because a snapshot was full: {recomputed}")
+
+
+def _small_scene(n, grid, seed, nb):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dx = 1.0 / grid
+    x0 = W.box_particles(n, [0.42, 0.1, 0.42], [0.58, 0.22, 0.58], seed, dx)
+    pos, rot, args = W.random_capsules(rng, nb, [0.5, 0.27, 0.5], spread=0.05, r_range=(0.02, 0.04),
+                                       h_range=(0.04, 0.08))
+    return W.make_scene(x0, grid, dx, 5e-5, W.capsule_prims(nb), args, pos, rot)
+
+
+def _oracle_rollout(sc, pose, gx):
+    osim = O.OracleSim(oracle_scene(sc), np.float32)
+    states = osim.rollout(initial_state(sc), pose[0], pose[1])
+    z = lambda *s: np.zeros(s, np.float32)  # noqa: E731
+    g_last = O.Grads(np.ascontiguousarray(gx, np.float32), z(sc.n, 3), z(sc.n, 3, 3), z(sc.n, 3, 3), z(sc.nb, 3),
+                     z(sc.nb, 4))
+    g0, pg, rg = osim.rollout_grad(states, g_last)
+    return states[-1].x, g0.x, pg, rg
+
+
+def test_small_batch_of_different_grids_matches_the_oracle():
+    T = 20
+    scenes = [_small_scene(1500, 32, 21, 2), _small_scene(2500, 48, 22, 4), _small_scene(1200, 64, 23, 3)]
+    poses = W.constant_velocity_poses(scenes, T, seed=24)
+    gx = _seed_grad(sum(s.n for s in scenes), seed=25)
+    eng, xT, g0, pg, rg, _ = _batched(scenes, poses, T, gx)
+    for i, sc in enumerate(scenes):
+        sl, bs = eng.scene_slice(i), eng.body_slice(i)
+        ox, og0, opg, org = _oracle_rollout(sc, poses[i], gx[sl])
+        dt = sc.spec.dt
+        e = dict(x=rel_err(xT[sl], ox), x0_grad=rel_err(g0[sl], og0),
+                 velocity_grad=rel_err(_velocity_grad(pg[:, bs], dt), _velocity_grad(opg, dt)),
+                 rot_grad_sum=rel_err(rg[:, bs].astype(np.float64).sum(0), org.astype(np.float64).sum(0)))
+        print(f"scene {i} (grid {sc.spec.grid_dim[0]}^3, {sc.n} particles): GPU batch vs oracle "
+              + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
+        assert e["x"] <= 1e-4, e
+        assert max(e["x0_grad"], e["velocity_grad"]) <= 1e-3 and e["rot_grad_sum"] <= 1e-2, e
+
+
+def _stream_requests():
+    stream = W.served_stream(50.0, 24, seed=41)
+    reqs = [Request(i, d["scene"].spec, d["scene"].x0, d["pose_pos"], d["pose_rot"], d["target"],
+                    arrival=d["arrival"], checkpoint_every=d["checkpoint_every"]) for i, d in enumerate(stream)]
+    # one small request (checked against the oracle) arriving mid-stream
+    sc = _small_scene(3000, 48, 42, 3)
+    T = 30
+    pos, rot = W.constant_velocity_poses([sc], T, seed=43)[0]
+    reqs.append(Request(len(reqs), sc.spec, sc.x0, pos, rot, W.plain_target(sc.x0, seed=44),
+                        arrival=reqs[len(reqs) // 2].arrival, checkpoint_every=10))
+    return reqs, sc
+
+
+def test_served_poisson_stream_mixed_equals_alone_and_small_request_matches_oracle():
+    reqs, small_sc = _stream_requests()
+    print(f"served: {len(reqs)} requests, particles {min(r.n for r in reqs)}-{max(r.n for r in reqs)}, "
+          f"horizons {min(r.T for r in reqs)}-{max(r.T for r in reqs)}", flush=True)
+    srv = RolloutServer(device=_dev())
+    mixed = srv.run(reqs)
+    stats = latency_stats(mixed)
+    events = sorted([(r.admitted, 1) for r in mixed.values()] + [(r.finished, -1) for r in mixed.values()])
+    live, most = 0, 0
+    for _, d in events:
+        live += d
+        most = max(most, live)
+    print(f"mixed: p50 {stats['p50_ms']:.1f} ms, p99 {stats['p99_ms']:.1f} ms, {srv.ticks} ticks, "
+          f"at most {most} requests in flight", flush=True)
+    assert most >= 3, "requests did not overlap"
+    assert all(r.finite for r in mixed.values())
+    ns = [r.n for r in reqs[:-1]]
+    Ts = [r.T for r in reqs[:-1]]
+    sample = sorted({int(np.argmin(ns)), int(np.argmax(ns)), int(np.argmax(Ts))})
+    for rid in sample:
+        r = reqs[rid]
+        alone = RolloutServer(device=_dev()).run([Request(r.rid, r.spec, r.x0, r.pose_pos, r.pose_rot, r.target,
+                                                          arrival=0.0, checkpoint_every=r.checkpoint_every)])[rid]
+        m = mixed[rid]
+        e = dict(loss=abs(m.loss - alone.loss) / abs(alone.loss),
+                 x=rel_err(m.x_final.cpu().numpy(), alone.x_final.cpu().numpy()),
+                 x0_grad=rel_err(m.x0_grad.cpu().numpy(), alone.x0_grad.cpu().numpy()),
+                 pos_grad=rel_err(m.pos_grad.cpu().numpy(), alone.pos_grad.cpu().numpy()),
+                 rot_grad=rel_err(m.rot_grad.cpu().numpy(), alone.rot_grad.cpu().numpy()))
+        print(f"request {rid} ({r.n} particles, grid {r.spec.grid_dim[0]}^3, T={r.T}): mixed vs alone "
+              + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
+        assert e["loss"] <= 1e-4 and e["x"] <= 1e-4, e
+        assert max(e["x0_grad"], e["pos_grad"], e["rot_grad"]) <= 1e-3, e
+    # the small request against the oracle (loss = mean |x_T - target|^2)
+    r = reqs[-1]
+    m = mixed[r.rid]
+    osim = O.OracleSim(oracle_scene(small_sc), np.float32)
+    states = osim.rollout(initial_state(small_sc), r.pose_pos, r.pose_rot)
+    d = states[-1].x.astype(np.float64) - r.target
+    loss = float((d * d).sum(-1).mean())
+    z = lambda *s: np.zeros(s, np.float32)  # noqa: E731
+    gl = O.Grads(((2.0 / r.n) * d).astype(np.float32), z(r.n, 3), z(r.n, 3, 3), z(r.n, 3, 3), z(small_sc.nb, 3),
+                 z(small_sc.nb, 4))
+    g0, pg, rg = osim.rollout_grad(states, gl)
+    dt = small_sc.spec.dt
+    e = dict(loss=abs(m.loss - loss) / loss, x=rel_err(m.x_final.cpu().numpy(), states[-1].x),
+             x0_grad=rel_err(m.x0_grad.cpu().numpy(), g0.x),
+             velocity_grad=rel_err(_velocity_grad(m.pos_grad.cpu().numpy(), dt), _velocity_grad(pg, dt)),
+             rot_grad_sum=rel_err(m.rot_grad.cpu().numpy().astype(np.float64).sum(0), rg.astype(np.float64).sum(0)))
+    print("small served request vs oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
+    assert e["loss"] <= 1e-4 and e["x"] <= 1e-4, e
+    assert max(e["x0_grad"], e["velocity_grad"]) <= 1e-3 and e["rot_grad_sum"] <= 1e-2, e
