@@ -22,9 +22,12 @@ c_void_p = ctypes.c_void_p
 
 DDM_EINVAL = 10001
 DDM_ETOOMANYBODIES = 10002
+DDM_ESCRATCH = 10003
 DDM_MAX_BODIES = 64
+DDM_SCRATCH_BLOCKS = 8192
 
-ERROR_NAMES = {DDM_EINVAL: "DDM_EINVAL (invalid argument)", DDM_ETOOMANYBODIES: "DDM_ETOOMANYBODIES"}
+ERROR_NAMES = {DDM_EINVAL: "DDM_EINVAL (invalid argument)", DDM_ETOOMANYBODIES: "DDM_ETOOMANYBODIES",
+               DDM_ESCRATCH: "DDM_ESCRATCH (more grid blocks than the fixed workspace holds)"}
 
 
 class DdmScene(ctypes.Structure):
@@ -98,7 +101,7 @@ REFERENCE_SYMBOLS = ["compute_grid_lower", "compute_svd", "compute_svd_grad", "p
 FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_forward_timed",
                  "ddm_substep_backward_timed", "ddm_substep_forward_taped", "ddm_substep_backward_taped",
                  "ddm_substep_forward_taped_timed", "ddm_substep_backward_taped_timed", "ddm_batch_reset",
-                 "ddm_version"]
+                 "ddm_version", "ddm_substep_forward_scratch"]
 SCORE_SYMBOLS = ["ddm_pairwise_workspace_bytes", "ddm_softmin", "ddm_plan_grad", "ddm_nearest"]
 ALL_SYMBOLS = RUNTIME_SYMBOLS + REFERENCE_SYMBOLS + FUSED_SYMBOLS + SCORE_SYMBOLS
 
@@ -140,6 +143,7 @@ _ARGTYPES = {
     "ddm_substep_forward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P],
     "ddm_substep_backward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
     "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],
+    "ddm_substep_forward_scratch": [ctypes.POINTER(DdmBatch), _P, _P, _P],
     "ddm_version": [],
     "ddm_softmin": [_P, _I, _P, _P, _I, _F, _I, _I, _P, _S, _P, _P],
     "ddm_plan_grad": [_P, _P, _I, _P, _P, _I, _F, _I, _I, _P, _S, _P, _P],

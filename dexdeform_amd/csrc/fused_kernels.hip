@@ -1,6 +1,8 @@
 // Fused multi-scene substep for gfx950: forward kernels (and the adjoint pass's
 // recompute and restore kernels), launch sequences and the exported entry points. The
 // layout and invariants are described in fused_common.h.
+#include <mutex>
+
 #include "fused_common.h"
 
 namespace ddm_fused {
@@ -681,6 +683,73 @@ DDM_API int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur,
                                        hipEvent_t *events) {
   if (!events) return DDM_EINVAL;
   return launch_backward(b, cur, next, cur_grad, next_grad, stream, events);
+}
+
+// ---------------------------------------------------------------- allocation-free forward
+// Grid workspace of ddm_substep_forward_scratch, in the library's own device memory: one
+// copy per GPU, allocated when the code object loads and held until the process ends.
+// Layout: (m, mv) [kScratchBlocks*64] float4 | v [kScratchBlocks*64] float4 |
+// flags [kScratchBlocks] u32 | list [kScratchBlocks][2] i32 | count (256 B).
+constexpr size_t kScratchBlocks = 8192;  // e.g. two 64^3 grids, or one 128x128x32 grid
+constexpr size_t kScratchNodes = kScratchBlocks * 64;
+constexpr size_t kScratchOffV = kScratchNodes * 16;
+constexpr size_t kScratchOffFlags = 2 * kScratchNodes * 16;
+constexpr size_t kScratchOffList = kScratchOffFlags + kScratchBlocks * 4;
+constexpr size_t kScratchOffCount = kScratchOffList + kScratchBlocks * 8;
+constexpr size_t kScratchBytes = kScratchOffCount + 256;  // 16.1 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per device (each GPU
+// holds its own copy at its own address; the array's name in host code is not that address).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_base(unsigned char **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return DDM_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!p) return DDM_EINVAL;
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+DDM_API int ddm_substep_forward_scratch(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                        hipStream_t stream) {
+  if (!b || b->n_blocks < 0) return DDM_EINVAL;
+  if ((size_t)b->n_blocks > kScratchBlocks) return DDM_ESCRATCH;
+  if (b->options & DDM_DETERMINISTIC) return DDM_EINVAL;  // its workspace is not part of the scratch
+  unsigned char *base = nullptr;
+  int e = scratch_base(&base);
+  if (e) return e;
+  ddm_batch sb = *b;
+  sb.grid_mmv = reinterpret_cast<float *>(base);
+  sb.grid_v = reinterpret_cast<float *>(base + kScratchOffV);
+  sb.block_flags = reinterpret_cast<uint32_t *>(base + kScratchOffFlags);
+  sb.block_list = reinterpret_cast<int32_t *>(base + kScratchOffList);
+  sb.block_count = reinterpret_cast<int32_t *>(base + kScratchOffCount);
+  sb.grid_dv = sb.grid_dmmv = nullptr;
+  sb.det = nullptr;
+  // The scratch keeps whatever the last call on this GPU left. A completed forward leaves
+  // (m, mv), flags and count cleared, but a failed or interrupted one may not: clear what
+  // must start at zero, for the blocks this launch addresses, with fill kernels.
+  const size_t nodes = (size_t)sb.n_blocks * 64;
+  if (nodes) {
+    k_zero<<<1024, kWG, 0, stream>>>(reinterpret_cast<float4 *>(sb.grid_mmv), nodes);
+    k_zero_u32<<<64, kWG, 0, stream>>>(sb.block_flags, (size_t)sb.n_blocks);
+  }
+  k_zero_u32<<<1, kWG, 0, stream>>>(reinterpret_cast<uint32_t *>(sb.block_count), 1);
+  e = (int)hipGetLastError();
+  if (e) return e;
+  return launch_forward(&sb, cur, next, stream, nullptr);
 }
 
 DDM_API int ddm_batch_reset(const ddm_batch *b, hipStream_t stream) {

@@ -37,6 +37,8 @@ extern "C" {
 /* error codes beyond hipError_t */
 #define DDM_EINVAL 10001   /* bad argument (null pointer, size, dims not multiple of 4) */
 #define DDM_ETOOMANYBODIES 10002
+#define DDM_ESCRATCH 10003 /* launch addresses more grid blocks than the fixed workspace of
+                              ddm_substep_forward_scratch holds (DDM_SCRATCH_BLOCKS) */
 
 /* ---------------------------------------------------------------- group 0
  * Device memory, copy and stream helpers under the names the reference's ctypes layer
@@ -287,6 +289,22 @@ int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, co
 int ddm_substep_backward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                      const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
                                      hipEvent_t *events);
+
+/* Allocation-free forward substep. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same work and results as ddm_substep_forward, but the
+ * grid workspace (the batch's grid_mmv, grid_v, block_flags, block_list, block_count,
+ * which are ignored and may be NULL) lives in a fixed array inside the library, one per
+ * GPU, held until the process ends. The caller still provides the scene, chunk and state
+ * tables and the particle arrays. Limits: b->n_blocks <= DDM_SCRATCH_BLOCKS (8192 node
+ * blocks of 4x4x4, e.g. two 64^3 grids; 16.1 MiB per GPU), else DDM_ESCRATCH; the
+ * deterministic mode is rejected (DDM_EINVAL). Each call clears the part of the workspace
+ * it uses with fill kernels on `stream` and allocates nothing. Every call on one GPU shares
+ * that GPU's workspace: calls that overlap on one GPU (different streams, or host threads)
+ * race and give wrong results; serialise them on one stream. */
+#define DDM_SCRATCH_BLOCKS 8192
+int ddm_substep_forward_scratch(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                hipStream_t stream);
 
 /* ---------------------------------------------------------------- group 3
  * Point-cloud scores between simulated particles and a target shape, computed without
