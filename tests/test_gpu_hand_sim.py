@@ -9,7 +9,8 @@ hand, model scale 1.5, fixed wrist, hand friction 0.9, 19 capsule/box geoms, qua
   frame and joint angles;
 * GradModel over two environment steps, L2 loss to a seeded target, loss.backward():
   action gradients against the FP32 oracle's adjoint chained through the same kinematics
-  on the CPU, and against the FP64 oracle's.
+  on the CPU, and against the FP64 oracle's: within 1e-4 of the FP32 oracle and 1e-3 of the
+  FP64 one.
 
 The hand configuration comes from tests/golden/shadow_rh_scale1p5.npz (the MJCF files are
 not needed on the GPU machine).
@@ -170,4 +171,6 @@ def test_hand_gradmodel_action_gradients_vs_oracle():
     print(f"loss {loss.item():.6e}; action grads: GPU vs oracle FP32 {e32:.2e}, vs oracle FP64 {e64:.2e} "
           f"(oracle FP32 vs FP64 {n32:.2e}); |g| {np.linalg.norm(g):.3e}")
     assert np.isfinite(g).all() and np.linalg.norm(g) > 0
-    assert e64 <= 1e-2 and e32 <= 1e-2
+    # measured: 2.0e-5 from the FP32 oracle, 2.9e-4 from the FP64 one (the FP32 oracle itself
+    # is 2.7e-4 from FP64: the FP32 state's floor over 80 substeps)
+    assert e32 <= 1e-4 and e64 <= 1e-3

@@ -8,8 +8,10 @@ and the CPU oracle (oracle/sinkhorn.py).
 * SamplesLoss("sinkhorn", p=1, blur=1e-4) -- the tutorial's score -- and p = 2, blur 0.05
   on 1,500 x 2,000 points: loss within 1e-5 relative of the oracle, gradient within 1e-4;
 * Chamfer distance and gradient: 1e-6;
-* at 100,000 x 200,000 points (the tutorial's sizes), where the oracle cannot run: the
-  divergence of a cloud with itself is 0 and of a translated copy is the translation length.
+* standard normal clouds at p = 1, blur 1e-4 (the tutorial's score) against the oracle at
+  1,000 x 1,000 (a translated copy) and 1,000 x 2,000;
+* at 100,000 x 200,000 points (the tutorial's sizes), where the oracle cannot run: S(x, x) = 0,
+  symmetry, invariance under permutation, homogeneity of degree 1 in the point scale.
 """
 import ctypes
 
@@ -52,8 +54,9 @@ def test_pair_kernels_against_dense_reductions(p, segments):
     print(f"pair kernels: n={n} m={m} p={p} segments={segments}", flush=True)
     for eps in (1e-4, 1e-2, 1.0):
         h = (rng.normal(size=m) * 3).astype(np.float32)
+        hd = _t(h)  # held for the launch: a freed temporary's block can be handed to the next allocation
         out = torch.empty(n, device=_dev())
-        _lib.check(lib.ddm_softmin(xd.data_ptr(), n, yd.data_ptr(), _t(h).data_ptr(), m, eps, p, segments,
+        _lib.check(lib.ddm_softmin(xd.data_ptr(), n, yd.data_ptr(), hd.data_ptr(), m, eps, p, segments,
                                    ws.data_ptr(), ws.numel(), out.data_ptr(), _lib.stream_ptr()))
         ref = _softmin_ref(x, y, h, eps, p)
         err = np.abs(out.cpu().numpy() - ref).max()
@@ -69,7 +72,8 @@ def test_pair_kernels_against_dense_reductions(p, segments):
             diff = np.where(r > 0, diff / np.where(r > 0, r, 1), 0)
         gref = (w[..., None] * diff).sum(1)
         g = torch.empty(n, 3, device=_dev())
-        _lib.check(lib.ddm_plan_grad(xd.data_ptr(), _t(a).data_ptr(), n, yd.data_ptr(), _t(b).data_ptr(), m, eps, p,
+        ad, bd = _t(a), _t(b)
+        _lib.check(lib.ddm_plan_grad(xd.data_ptr(), ad.data_ptr(), n, yd.data_ptr(), bd.data_ptr(), m, eps, p,
                                      segments, ws.data_ptr(), ws.numel(), g.data_ptr(), _lib.stream_ptr()))
         e = rel_err(g.cpu().numpy(), gref)
         print(f"  plan_grad eps {eps:g}: rel err {e:.2e}")
@@ -131,19 +135,61 @@ def test_chamfer_matches_oracle():
     assert rel_err(xd.grad.cpu().numpy(), g) <= 1e-6
 
 
+@pytest.mark.parametrize("m", [1000, 2000])
+def test_gaussian_clouds_at_the_tutorials_parameters_match_oracle(m):
+    """p = 1, blur 1e-4 on standard normal clouds (the tutorial's inputs, at sizes the
+    oracle finishes in seconds): y a translated copy (m = 1000) or an independent cloud."""
+    rng = np.random.default_rng(40 + m)
+    x = rng.normal(size=(1000, 3)).astype(np.float32)
+    y = (x + np.float32([0.3, -0.2, 0.1])) if m == 1000 else rng.normal(size=(m, 3)).astype(np.float32)
+    xd = _t(x).requires_grad_(True)
+    S = L.SamplesLoss("sinkhorn", p=1, blur=1e-4)(xd, _t(y))
+    S.backward()
+    S_ref, g_ref, _ = SK.sinkhorn(x.astype(np.float64), y.astype(np.float64), p=1, blur=1e-4)
+    e_s = abs(S.item() - S_ref) / abs(S_ref)
+    e_g = rel_err(xd.grad.cpu().numpy(), g_ref)
+    print(f"gaussian 1000x{m}: S {S.item():.8e} (oracle {S_ref:.8e}) rel err {e_s:.2e}; grad rel err {e_g:.2e}")
+    # the plan weights exp((f_i + g_j - C_ij) / eps) resolve potentials of the clouds' size
+    # (diameter ~8 here) at eps = 1e-4 in FP32: ~diameter / eps * 2^-24 = 5e-3 per weight,
+    # 1.6e-4 to 1.8e-4 on the gradient (4e-5 on the 0.5-wide uniform clouds above)
+    assert e_s <= 1e-5 and e_g <= 1e-3
+
+
 def test_tutorial_sized_score_properties():
-    """100,000 x 200,000 points, p = 1, blur 1e-4 (tutorials/4_computing_score.ipynb cell 3)."""
+    """100,000 x 100,000 and 100,000 x 200,000 standard normal points, p = 1, blur 1e-4
+    (tutorials/4_computing_score.ipynb cell 3), where the oracle cannot run; properties that
+    hold for the algorithm at any size:
+
+    * S(x, x) = 0 (the debiasing terms equal the cross terms);
+    * S(x, y) = S(y, x) and S(x[perm], y) = S(x, y) up to FP32 summation order;
+    * S(2x, 2y) with blur 2e-4 = 2 S(x, y) with blur 1e-4 (the p = 1 cost, the diameter and
+      the epsilon schedule all scale by 2);
+    * a finite, non-zero gradient at 100,000 x 200,000.
+
+    S(x, x + t) is not |t| here: one update per epsilon of the schedule is far from converged
+    at this size (the oracle gives 0.306 at 1,000 points and 0.245 at 3,000 for |t| = 0.374,
+    and the value keeps falling with the point count), so the translation distance is
+    checked only where the oracle runs."""
     g = torch.Generator(device="cpu").manual_seed(0)
     x = torch.randn(100000, 3, generator=g).to(_dev())
     loss = L.SamplesLoss("sinkhorn", p=1, blur=1e-4)
     s0 = loss(x, x.clone()).item()
     t = torch.tensor([0.3, -0.2, 0.1], device=_dev())
-    st = loss(x, x + t).item()
-    print(f"S(x, x) = {s0:.3e}; S(x, x + t) = {st:.6f} vs |t| = {t.norm().item():.6f}", flush=True)
+    y = x + t
+    sxy, syx = loss(x, y).item(), loss(y, x).item()
+    perm = torch.randperm(100000, generator=g).to(_dev())
+    sp = loss(x[perm], y).item()
+    s2 = L.SamplesLoss("sinkhorn", p=1, blur=2e-4)(2 * x, 2 * y).item()
+    print(f"S(x, x) = {s0:.3e}; S(x, x + t) = {sxy:.7f}, S(x + t, x) = {syx:.7f}, permuted {sp:.7f}, "
+          f"scaled by 2: {s2:.7f} (|t| = {t.norm().item():.6f})", flush=True)
     assert abs(s0) <= 1e-5
-    assert abs(st - t.norm().item()) <= 1e-3 * t.norm().item()
+    assert sxy > 0
+    assert abs(syx - sxy) <= 1e-4 * sxy
+    assert abs(sp - sxy) <= 1e-4 * sxy
+    assert abs(s2 - 2 * sxy) <= 1e-4 * 2 * sxy
     y = torch.randn(200000, 3, generator=g).to(_dev())
     xg = x.clone().requires_grad_(True)
     s = loss(xg, y)
     s.backward()
     assert torch.isfinite(s) and torch.isfinite(xg.grad).all() and s.item() > 0
+    assert xg.grad.norm().item() > 0

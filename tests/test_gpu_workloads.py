@@ -13,12 +13,14 @@ sampled requests against their stand-alone results, and the small one against th
 
 Tolerances: 1e-4 relative (L2) for final positions and losses, 1e-3 for gradients; FP32
 atomic accumulation order is the difference between the runs. Body-pose gradients are
-compared per substep between GPU runs, and against the oracle through what an action sees:
+compared, between GPU runs and against the oracle, through what an action sees:
 the gradient of each capsule's constant velocity (sum over substeps of t dt dL/dpos_t)
 within 1e-3, and the rotation gradient summed over substeps within 1e-2: each substep's
 rotation gradient carries terms of order |dL/dv| / dt that cancel between consecutive
 substeps, so FP32 summation order alone moves the sum by a few 1e-3 (the FP32 oracle is
 4.0e-3 from the FP64 oracle on the 32^3 scene below; the GPU 2.1e-3 from the FP32 oracle).
+Per-substep pose gradients between GPU runs are held to 1e-2 for the same reason (a
+served request with T = 200 differs from itself run alone by 1.5e-3 in them).
 """
 import numpy as np
 import pytest
@@ -183,15 +185,21 @@ def test_served_poisson_stream_mixed_equals_alone_and_small_request_matches_orac
         alone = RolloutServer(device=_dev()).run([Request(r.rid, r.spec, r.x0, r.pose_pos, r.pose_rot, r.target,
                                                           arrival=0.0, checkpoint_every=r.checkpoint_every)])[rid]
         m = mixed[rid]
+        mpg, apg = m.pos_grad.cpu().numpy(), alone.pos_grad.cpu().numpy()
+        mrg, arg = m.rot_grad.cpu().numpy().astype(np.float64), alone.rot_grad.cpu().numpy().astype(np.float64)
         e = dict(loss=abs(m.loss - alone.loss) / abs(alone.loss),
                  x=rel_err(m.x_final.cpu().numpy(), alone.x_final.cpu().numpy()),
                  x0_grad=rel_err(m.x0_grad.cpu().numpy(), alone.x0_grad.cpu().numpy()),
-                 pos_grad=rel_err(m.pos_grad.cpu().numpy(), alone.pos_grad.cpu().numpy()),
-                 rot_grad=rel_err(m.rot_grad.cpu().numpy(), alone.rot_grad.cpu().numpy()))
+                 velocity_grad=rel_err(_velocity_grad(mpg, r.spec.dt), _velocity_grad(apg, r.spec.dt)),
+                 rot_grad_sum=rel_err(mrg.sum(0), arg.sum(0)),
+                 pos_grad=rel_err(mpg, apg), rot_grad=rel_err(mrg, arg))
         print(f"request {rid} ({r.n} particles, grid {r.spec.grid_dim[0]}^3, T={r.T}): mixed vs alone "
               + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
         assert e["loss"] <= 1e-4 and e["x"] <= 1e-4, e
-        assert max(e["x0_grad"], e["pos_grad"], e["rot_grad"]) <= 1e-3, e
+        # per-substep pose gradients carry the |dL/dv| / dt terms that cancel between
+        # substeps: accumulation order alone moves them by ~1.5e-3 at T = 200
+        assert max(e["x0_grad"], e["velocity_grad"]) <= 1e-3 and e["rot_grad_sum"] <= 1e-2, e
+        assert max(e["pos_grad"], e["rot_grad"]) <= 1e-2, e
     # the small request against the oracle (loss = mean |x_T - target|^2)
     r = reqs[-1]
     m = mixed[r.rid]
