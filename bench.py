@@ -297,6 +297,77 @@ def score_benchmark(dev, n=100000, m=200000, reps=3):
     return out
 
 
+def render_benchmark(dev, steps=20):
+    """Random-action stepping with a render per step, the loop of
+    tutorials/2_environment_loading.ipynb cell 5 (reference: 11.52 it/s on its GPU, 100 steps
+    of the folding scene: 10,000 particles, 40 substeps per step, a 128 x 128 image with
+    spp 10 per step through Viewer.render): the drop-in HandSimulator with the stored
+    Shadow hand (19 geoms, scale 1.5), a flat cylinder of 10,000 particles (r 0.16, h 0.01,
+    the folding shape), E 4000, yield stress 250, ground friction 500, gravity (0, -10, 0);
+    per step one random action, 40 forward substeps, bake + render (voxel box 168^3, two
+    bounces) and the uint8 image on the host. Also the render alone at 128^2 spp 10 and at
+    the Renderer's defaults (512^2, spp 3), and the bake alone."""
+    import os
+    import torch
+    from dexdeform_amd.mpm import hand as HM
+    from dexdeform_amd.mpm.renderer import Renderer
+    cfg_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "shadow_rh_scale1p5.npz")
+    prims, hand_cfg = HM.load_hand_config(cfg_path)
+    nb, kw = HM.hand_bodies(prims)
+    n = 10000
+    sim = HM.HandSimulator(nb, hand_cfg, cfg=dict(n_particles=n, E=4e3, nu=0.2, yield_stress=250.0,
+                                                  ground_friction=500.0, quality=1, max_steps=steps * 40 + 81,
+                                                  gravity=(0.0, -10.0, 0.0), fixed_base=True), device=dev)
+    sim.init_bodies(**kw)
+    rng = np.random.Generator(np.random.PCG64(5))
+    r = 0.16 * np.sqrt(rng.uniform(0, 1, n))
+    a = rng.uniform(0, 2 * np.pi, n)
+    x0 = np.stack([0.5 + r * np.cos(a), 0.056875 + rng.uniform(0, 0.01, n), 0.6 + r * np.sin(a)], 1)
+    base = HM.root_matrix([0.5, 0.2, 0.3], [0.0, 0.0, 0.0])
+    sim.set_state(0, (x0.astype(np.float32), np.zeros((n, 3), np.float32),
+                      np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)), np.zeros((n, 3, 3), np.float32), base[None],
+                      np.zeros((1, 24))))
+    sim.set_color(np.zeros(n, np.int64) + 0xFFFFFF)
+    side = dict(center=(1.2, 0.35, 0.6), theta=np.pi / 8, phi=-np.pi / 2, radius=1.5)  # Viewer's side view
+    small = Renderer(image_res=(128, 128), device=dev)
+    small.lookat(**side)
+    acts = torch.tensor(rng.uniform(0, 1, (steps + 2, 1, 26)), dtype=torch.float32, device=dev)
+
+    def step(i):
+        sim.step(acts[i])
+        return np.uint8(small.render(sim, sim.states[0], spp=10)[..., :3].clip(0, 1) * 255)
+
+    step(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        img = step(i + 1)
+    torch.cuda.synchronize()
+    it_s = steps / (time.perf_counter() - t0)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    big = Renderer(device=dev)
+    big.lookat(**side)
+    out = {"scene": "stored Shadow hand (19 geoms) + 10k-particle cylinder, 40 substeps per step",
+           "step_and_render_it_per_s": it_s, "reference_it_per_s": 11.52,
+           "render_128_spp10_ms": 1e3 * timed(lambda: small.render(sim, sim.states[0], to_numpy=False, spp=10), 10),
+           "render_512_spp3_ms": 1e3 * timed(lambda: big.render(sim, sim.states[0], to_numpy=False), 5),
+           "bake_168_ms": 1e3 * timed(lambda: small.bake_particles(sim, sim.states[0]), 10),
+           "image_mean": float(img.mean())}
+    log(f"render: step + 128^2 spp-10 render {it_s:.1f} it/s (reference 11.52); render alone "
+        f"{out['render_128_spp10_ms']:.2f} ms (128^2 spp 10), {out['render_512_spp3_ms']:.2f} ms (512^2 spp 3), "
+        f"bake {out['bake_168_ms']:.2f} ms")
+    return out
+
+
 def run_step(ro, gx):
     ro.forward()
     ro.backward(x_grad=gx)
@@ -480,6 +551,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-score", action="store_true", help="skip the shape-score (Sinkhorn, Chamfer) leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the step + render leg")
     ap.add_argument("--no-tape", action="store_true",
                     help="adjoint recomputes P2G and the grid update (as the reference does) instead of "
                          "reading the forward pass's tape")
@@ -669,6 +741,7 @@ def main():
                                 not args.serve_eager)
 
     score = None if args.no_score else score_benchmark(dev)
+    render = None if args.no_render else render_benchmark(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -727,6 +800,7 @@ def main():
         "trajectory_optimisation": tutorial,
         "serve": serve,
         "score": score,
+        "render": render,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -68,6 +68,11 @@ class DdmDet(ctypes.Structure):
 assert ctypes.sizeof(DdmDet) == 80
 
 
+class DdmTextureResources(ctypes.Structure):
+    """(hipArray_t, hipTextureObject_t) returned by create_volume (types.py's texture_resources)."""
+    _fields_ = [("array", c_void_p), ("texture", c_void_p)]
+
+
 class DdmBatch(ctypes.Structure):
     _fields_ = [
         ("n_scenes", c_int32), ("n_chunks", c_int32), ("n_blocks", c_int32), ("max_bodies", c_int32),
@@ -103,7 +108,8 @@ FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_for
                  "ddm_substep_forward_taped_timed", "ddm_substep_backward_taped_timed", "ddm_batch_reset",
                  "ddm_version", "ddm_substep_forward_scratch"]
 SCORE_SYMBOLS = ["ddm_pairwise_workspace_bytes", "ddm_softmin", "ddm_plan_grad", "ddm_nearest"]
-ALL_SYMBOLS = RUNTIME_SYMBOLS + REFERENCE_SYMBOLS + FUSED_SYMBOLS + SCORE_SYMBOLS
+RENDER_SYMBOLS = ["particle_sdf", "render", "create_volume", "destroy_volume"]
+ALL_SYMBOLS = RUNTIME_SYMBOLS + REFERENCE_SYMBOLS + FUSED_SYMBOLS + SCORE_SYMBOLS + RENDER_SYMBOLS
 
 _P = c_void_p
 _I = ctypes.c_int
@@ -148,6 +154,10 @@ _ARGTYPES = {
     "ddm_softmin": [_P, _I, _P, _P, _I, _F, _I, _I, _P, _S, _P, _P],
     "ddm_plan_grad": [_P, _P, _I, _P, _P, _I, _F, _I, _I, _P, _S, _P, _P],
     "ddm_nearest": [_P, _I, _P, _I, _I, _P, _S, _P, _P, _P],
+    # ivec3 arguments: pointers to 3 host ints (as types.py:284-286 passes POINTER(ivec3))
+    "particle_sdf": [_P] * 5 + [_I, _P, _F] + [_P] * 3 + [_I, _P],
+    "render": [_P] * 13 + [_F, _P, _I, ctypes.c_bool, _P, _I, _I, _F, _I, _P, _P],
+    "create_volume": [_P, _I, _I, _I],
 }
 
 _lib = None
@@ -178,6 +188,9 @@ def load(path: str = LIB_PATH):
     lib.print_memory_info.restype = None
     lib.ddm_pairwise_workspace_bytes.argtypes = [_I, _I]
     lib.ddm_pairwise_workspace_bytes.restype = _S
+    lib.create_volume.restype = DdmTextureResources
+    lib.destroy_volume.argtypes = [DdmTextureResources]
+    lib.destroy_volume.restype = ctypes.c_int
     _lib = lib
     return lib
 

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libdexdeform_mpm.so"
-SOURCES = ["runtime_api.hip", "stage_kernels.hip", "fused_kernels.hip", "fused_adjoint.hip", "pairwise.hip"]
+SOURCES = ["runtime_api.hip", "stage_kernels.hip", "fused_kernels.hip", "fused_adjoint.hip", "pairwise.hip",
+           "render.hip"]
 # per-source flags added after HIPCC_FLAGS (a later -ffp-contract wins): the fused adjoint
 # kernels feed only gradients (1e-3 bar) and may contract a*b+c into FMAs
 SOURCE_FLAGS = {"fused_adjoint.hip": ["-ffp-contract=fast"]}

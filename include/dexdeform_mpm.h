@@ -19,7 +19,7 @@
  *     (mpm/simulator.py:561-571) and ddm_substep_backward replaces
  *     MPMSimulator.substep_grad (mpm/simulator.py:573-585), for every scene at once.
  *
- * Groups 1 and 2 enqueue all work on the given stream and neither synchronise the device
+ * Groups 1 to 4 enqueue all work on the given stream and neither synchronise the device
  * nor allocate device memory, so their sequences can be captured in a HIP graph (group 0
  * allocates, copies and synchronises, as the reference's helpers do).
  */
@@ -28,6 +28,9 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 #include <hip/hip_runtime_api.h>
 
 #ifdef __cplusplus
@@ -326,6 +329,42 @@ int ddm_plan_grad(const float *x, const float *a, int n, const float *y, const f
 /* dist2[i] = min_j |x_i - y_j|^2, index[i] = its j (the smallest on ties) */
 int ddm_nearest(const float *x, int n, const float *y, int m, int segments, void *workspace, size_t workspace_bytes,
                 float *dist2, int32_t *index, hipStream_t stream);
+
+/* ---------------------------------------------------------------- group 4
+ * Renderer: the reference's launchers particle_sdf (integrator.cu:1908-1933) and render
+ * (integrator.cu:1863-1906), bound by mpm/types.py:284-286 and called by
+ * Renderer.bake_particles / Renderer.render (mpm/renderer.py:89-169). Same names,
+ * argument order and meaning; pointers are device pointers (vec3 = 3 floats, mat3 = 9
+ * floats row-major, quat = 4 floats (w, x, y, z)), `ivec3 const&` becomes `const int *`
+ * to 3 host ints. Unlike the reference's render, which cudaMallocs and frees one curand
+ * state per pixel on every call, render allocates nothing: its random numbers are a
+ * counter-based Philox4x32-10 stream per (pixel, sample) keyed by `seed`, so its images
+ * are not the reference's bit for bit (different random numbers; same estimator).
+ * n_bodies <= DDM_MAX_BODIES. */
+/* volume: int64 per voxel, filled with (1 << 33) - 1 by the caller; bbox_max unused (as in
+ * the reference); sdf, sdf_tmp: float per voxel; color: vec3 per voxel. */
+int particle_sdf(long long *volume, float *particle_x, int *particle_color, float *bbox_min, float *bbox_max,
+                 int bake_size, const int *grid_dim, float inv_dx, float *sdf, float *color, float *sdf_tmp,
+                 int n_particles, hipStream_t stream);
+/* color_buffer: vec3 per pixel, depth_buffer: float per pixel, pixel p = row * W + column
+ * with image_dim = (H, W, 3); both are overwritten. */
+int render(float *sdf_volume, float *box_min, float *box_max, float *color_volume, float *body_pos, float *body_rot,
+           float *body_type_friction_softness_round, float *body_args, float *camera_rot, float *camera_pos,
+           float *camera_intrinsic, float *color_buffer, float *depth_buffer, float sdf_threshold,
+           const int *grid_dim, int n_bodies, bool visualize_shape, const int *image_dim, int max_ray_depth, int spp,
+           float ground_height, int seed, float *light_direction, hipStream_t stream);
+
+/* create_volume / destroy_volume (integrator.cu:2074-2124, bound by mpm/types.py:121-124):
+ * a float4 volume in host memory (x fastest, then y, z) copied into a 3-D array with a
+ * linearly filtered, clamped texture object. They allocate and free, as the reference's
+ * do; nothing in the reference samples the texture. On failure create_volume returns
+ * null handles; destroy_volume returns 0 or the first hipError_t. */
+typedef struct {
+  hipArray_t array;
+  hipTextureObject_t texture;
+} ddm_texture_resources; /* 16 bytes: the layout of types.py's texture_resources */
+ddm_texture_resources create_volume(float *data, int x, int y, int z);
+int destroy_volume(ddm_texture_resources tex);
 
 /* Zeroes the batch's grid arenas and block flags (after an error or on setup). */
 int ddm_batch_reset(const ddm_batch *b, hipStream_t stream);

@@ -1052,6 +1052,295 @@ void g2p_grad(int n, const T *x, const T *grid_v, const Grid &g, T dx, T inv_dx,
   }
 }
 
+
+// ======================================================================== renderer
+// CPU restatement of the renderer (FP32): particle_sdf (bake_volume, copy_volume and two
+// smooth passes, integrator.cu:1438-1525,1908-1933) and render (ray_aabb_intersection,
+// next_hit, render, integrator.cu:1111-1436; volume queries array.h:6-75). The reference
+// draws its jitter from curand's XORWOW stream per pixel (integrator.cu:1306); this
+// restatement, like the HIP renderer it checks, draws from Philox4x32-10 (Salmon, Moraes,
+// Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11) with one stream per
+// (pixel, sample): draw k of sample s of pixel p is word k % 4 of
+// Philox(counter = (k / 4, p, s, 0), key = (seed, 0)), mapped to (0, 1] as
+// ((w >> 8) + 1) 2^-24. cos/sin of the bounce angle: Taylor series after an exact
+// reduction (see sincos_taylor). Samples are traced in order and added as the reference's
+// loop adds them.
+namespace rnd {
+
+inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t o0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t o1 = (uint32_t)p1;
+    uint32_t o2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    uint32_t o3 = (uint32_t)p0;
+    c[0] = o0; c[1] = o1; c[2] = o2; c[3] = o3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+struct Stream {
+  uint32_t seed, pixel, sample, block = 0, used = 4, w[4] = {0, 0, 0, 0};
+  float uniform() {
+    if (used == 4) {
+      w[0] = block++; w[1] = pixel; w[2] = sample; w[3] = 0;
+      philox4x32_10(w, seed, 0u);
+      used = 0;
+    }
+    uint32_t x = w[used++];
+    return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// sin and cos of phi in [0, 2 pi]: k = round(phi 2/pi), r = phi - k pi/2 with pi/2 =
+// 1.5703125 + 4.83826794e-4 in two fused steps, then Taylor polynomials (sin to r^9, cos
+// to r^10) in Horner form with fused multiply-adds; quadrant by k mod 4.
+inline void sincos_taylor(float phi, float &s, float &c) {
+  float k = std::rint(phi * 0.636619772f);
+  float r = std::fmaf(-k, 1.5703125f, phi);
+  r = std::fmaf(-k, 4.83826794e-4f, r);
+  float r2 = r * r;
+  float ps = std::fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);
+  ps = std::fmaf(r2, ps, 8.33333377e-3f);
+  ps = std::fmaf(r2, ps, -1.66666672e-1f);
+  float sn = std::fmaf(r * r2, ps, r);
+  float pc = std::fmaf(r2, -2.75573188e-7f, 2.48015876e-5f);
+  pc = std::fmaf(r2, pc, -1.38888892e-3f);
+  pc = std::fmaf(r2, pc, 4.16666679e-2f);
+  pc = std::fmaf(r2, pc, -0.5f);
+  float cs = std::fmaf(r2, pc, 1.0f);
+  switch (((int)k) & 3) {
+    case 0: s = sn; c = cs; break;
+    case 1: s = cs; c = -sn; break;
+    case 2: s = -sn; c = -cs; break;
+    default: s = -cs; c = sn; break;
+  }
+}
+
+}  // namespace rnd
+
+namespace rdr {
+using F3 = V3<float>;
+
+struct Vol {
+  const float *sdf, *color;
+  F3 lo, hi;
+  int d[3];
+};
+
+// volume_query (array.h:6-49) for C channels
+inline bool volume_query(const Vol &V, const float *values, int C, F3 point, float *out) {
+  F3 ratio{(point.x - V.lo.x) / (V.hi.x - V.lo.x), (point.y - V.lo.y) / (V.hi.y - V.lo.y),
+           (point.z - V.lo.z) / (V.hi.z - V.lo.z)};
+  if (!(ratio.x >= 0 && ratio.y >= 0 && ratio.z >= 0 && ratio.x < 1. && ratio.y < 1. && ratio.z < 1.)) return false;
+  F3 grid{ratio.x * (float)V.d[0], ratio.y * (float)V.d[1], ratio.z * (float)V.d[2]};
+  int x0 = std::min(std::max((int)std::floor(grid.x), 0), V.d[0] - 1);
+  int y0 = std::min(std::max((int)std::floor(grid.y), 0), V.d[1] - 1);
+  int z0 = std::min(std::max((int)std::floor(grid.z), 0), V.d[2] - 1);
+  float dx = grid.x - (float)x0, dy = grid.y - (float)y0, dz = grid.z - (float)z0;
+  int base = (x0 * V.d[1] + y0) * V.d[2] + z0;
+  int fz = z0 + 1 < V.d[2], fy = y0 + 1 < V.d[1], fx = x0 + 1 < V.d[0];
+  int idx[8];
+  idx[0] = base;
+  idx[1] = base + fz;
+  idx[2] = base + V.d[2] * fy;
+  idx[3] = idx[2] + fz;
+  idx[4] = base + V.d[1] * V.d[2] * fx;
+  idx[5] = idx[4] + fz;
+  idx[6] = idx[4] + V.d[2] * fy;
+  idx[7] = idx[6] + fz;
+  float wx[2] = {1 - dx, dx}, wy[2] = {1 - dy, dy}, wz[2] = {1 - dz, dz};
+  for (int c = 0; c < C; ++c) {
+    float acc = 0.f;
+    for (int t = 0; t < 8; ++t) {
+      float v = values[C * idx[t] + c] * wx[t >> 2] * wy[(t >> 1) & 1] * wz[t & 1];
+      acc = t == 0 ? v : acc + v;
+    }
+    out[c] = acc;
+  }
+  return true;
+}
+inline float sample_sdf(const Vol &V, F3 p, float thr) {  // array.h:51-59
+  float out = 0.f;
+  if (volume_query(V, V.sdf, 1, p, &out)) out -= thr;
+  return out;
+}
+inline F3 sample_normal(const Vol &V, F3 p) {  // array.h:61-70
+  float e = 1e-3f;
+  float x0 = sample_sdf(V, p + F3{-e, 0.f, 0.f}, 0.f), x1 = sample_sdf(V, p + F3{e, 0.f, 0.f}, 0.f);
+  float y0 = sample_sdf(V, p + F3{0.f, -e, 0.f}, 0.f), y1 = sample_sdf(V, p + F3{0.f, e, 0.f}, 0.f);
+  float z0 = sample_sdf(V, p + F3{0.f, 0.f, -e}, 0.f), z1 = sample_sdf(V, p + F3{0.f, 0.f, e}, 0.f);
+  return F3{x1 - x0, y1 - y0, z1 - z0} * (float)(0.5 / (double)e);
+}
+inline F3 sample_color(const Vol &V, F3 p) {  // array.h:72-76
+  float c[3] = {0.f, 0.f, 0.f};
+  volume_query(V, V.color, 3, p, c);
+  return F3{c[0], c[1], c[2]};
+}
+// ray_aabb_intersection (integrator.cu:1111-1143)
+inline bool ray_aabb(F3 lo, F3 hi, F3 o, F3 d, float &tn, float &tf) {
+  float dd[3] = {d.x, d.y, d.z}, oo[3] = {o.x, o.y, o.z}, bl[3] = {lo.x, lo.y, lo.z}, bh[3] = {hi.x, hi.y, hi.z};
+  tn = -1e9f;
+  tf = 1e9f;
+  for (int i = 0; i < 3; ++i) {
+    if (dd[i] == 0) {
+      if (oo[i] < bl[i] || oo[i] > bh[i]) return false;
+    } else {
+      float i1 = (bl[i] - oo[i]) / dd[i], i2 = (bh[i] - oo[i]) / dd[i];
+      tf = std::min(std::max(i1, i2), tf);
+      tn = std::max(std::min(i1, i2), tn);
+    }
+  }
+  return !(tn > tf);
+}
+
+struct Scene {
+  Vol V;
+  float thr, ground;
+  int nb;
+  bool shape;
+  const float *pos, *rot, *tfsr, *args;
+};
+
+// next_hit (integrator.cu:1145-1267)
+inline float next_hit(const Scene &S, F3 o, F3 d, F3 &color, F3 &normal, int &material) {
+  const float limit = 100.f;
+  float closest = 1e9f;
+  if (d.y < 0) {
+    float gd = (o.y - S.ground) / (-d.y);
+    if (gd < limit && gd < closest) {
+      closest = gd;
+      normal = F3{0.f, 1.f, 0.f};
+      color = F3{0.3f, 0.5f, 0.7f};
+      F3 p = o + d * closest;
+      if (p.x <= 1 && p.x >= 0 && p.z <= 1 && p.z >= 0)
+        color = color * (float)((((int)(p.x / 0.25) + (int)(p.z / 0.25)) % 2) * 0.2 + 0.35);
+      else
+        color = color * 0.4f;
+      material = 1;
+    }
+  }
+  if (S.nb > 0) {
+    int j = 0, id = 0;
+    float dist = 0.f, val = 1e9f;
+    F3 pp{0.f, 0.f, 0.f};
+    while (j < 200 && dist < limit && val > 1e-8) {
+      pp = o + d * dist;
+      val = 1e9f;
+      for (int i = 0; i < S.nb; ++i) {
+        F3 gb = xform_inv(load_v(S.pos + 3 * i), load_q(S.rot + 4 * i), pp);
+        float t = shape_sdf(S.tfsr + 4 * i, S.args + 4 * i, gb);
+        if (t < val) {
+          val = t;
+          id = i;
+        }
+      }
+      dist += val;
+      j += 1;
+    }
+    if (dist < closest && dist < limit) {
+      closest = dist;
+      Q4<float> q = load_q(S.rot + 4 * id);
+      F3 gb = xform_inv(load_v(S.pos + 3 * id), q, pp);
+      normal = qrot(q, normalized(shape_grad(S.tfsr + 4 * id, S.args + 4 * id, gb)));
+      color = F3{0.5f, 0.5f, 0.5f};
+      material = 1;
+    }
+  }
+  float tn = 0.f, tf = 0.f;
+  if (S.shape && ray_aabb(S.V.lo, S.V.hi, o, d, tn, tf)) {
+    tn = (float)std::max((double)tn, 0.);
+    F3 pos = o + d * (float)((double)tn + 1e-4);
+    F3 step{0.f, 0.f, 0.f};
+    for (int j = 0; j < 500; ++j) {
+      float s = sample_sdf(S.V, pos, S.thr);
+      if (s < 0) {
+        F3 back = step;
+        for (int k = 0; k < 20; ++k) {
+          back = back * 0.5f;
+          if (sample_sdf(S.V, pos - back, S.thr) < 0) pos = pos - back;
+        }
+        float dist = std::sqrt(dot(o - pos, o - pos));
+        if (dist < closest) {
+          closest = dist;
+          normal = normalized(sample_normal(S.V, pos));
+          color = sample_color(S.V, pos);
+          material = 1;
+        }
+        break;
+      }
+      step = d * (float)std::max((double)s * 0.05, 0.01);
+      pos += step;
+    }
+  }
+  return closest;
+}
+
+inline F3 matvec(const float *m, F3 b) {  // mat3::mul (mat3.h:95-98)
+  F3 c0{m[0], m[3], m[6]}, c1{m[1], m[4], m[7]}, c2{m[2], m[5], m[8]};
+  return c0 * b.x + c1 * b.y + c2 * b.z;
+}
+
+// one sample (integrator.cu:1315-1413 with use_directional_light = 1, noise 0.1)
+inline void trace(const Scene &S, const float *K, const float *R, F3 cpos, F3 light, int depth_max, int u, int v,
+                  rnd::Stream &rs, F3 &out, float &first) {
+  float ju = rs.uniform();
+  float jv = rs.uniform();
+  F3 dir{(float)u + ju, (float)v + jv, 1.f};
+  F3 contrib{0.f, 0.f, 0.f}, through{1.f, 1.f, 1.f};
+  first = 0.f;
+  dir = matvec(K, dir);
+  F3 d = matvec(R, normalized(dir));
+  F3 pos = cpos;
+  int depth = 0;
+  while (depth < depth_max) {
+    F3 normal{0.f, 0.f, 0.f}, color{0.f, 0.f, 0.f};
+    int material = 0;
+    float closest = next_hit(S, pos, d, color, normal, material);
+    if (depth == 0) first = closest;
+    F3 hit = pos + d * closest;
+    depth += 1;
+    if (dot(normal, normal) > 1e-10) {
+      F3 odir;
+      if (material == 0) {
+        odir = d - normal * (dot(d, normal) * 2.f);
+      } else {
+        F3 ub{1.f, 0.f, 0.f};
+        if (std::fabs(normal.y) < 1 - 1e-3) ub = normalized(cross(normal, F3{0.f, 1.f, 0.f}));
+        F3 vb = cross(normal, ub);
+        float phi = (float)(2 * 3.14159265358979323846 * (double)rs.uniform());
+        float r = rs.uniform();
+        float ay = std::sqrt(r), ax = std::sqrt(1 - r);
+        float sp, cp;
+        rnd::sincos_taylor(phi, sp, cp);
+        odir = (ub * cp + vb * sp) * ax + normal * ay;
+      }
+      d = normalized(odir + F3{0.f, 0.f, 0.f});
+      pos = hit + d * 1e-4f;
+      through = through * color;
+      float n0 = rs.uniform() - 0.5f;
+      float n1 = rs.uniform() - 0.5f;
+      float n2 = rs.uniform() - 0.5f;
+      F3 direct = normalized(light + F3{n0, n1, n2} * 0.1f);
+      float dv = dot(direct, normal);
+      if (dv > 0) {
+        F3 c2 = color, nn = normal;
+        int m2 = material;
+        float dist = next_hit(S, pos, direct, c2, nn, m2);
+        if (dist > 100.f) contrib += through * dv * F3{0.8f, 0.8f, 0.8f};
+      }
+    } else {
+      break;
+    }
+  }
+  contrib += through * F3{0.8f, 0.8f, 0.8f};
+  out = contrib;
+}
+
+}  // namespace rdr
+
 }  // namespace orc
 
 // ======================================================================== C ABI
@@ -1159,6 +1448,130 @@ void g2p_grad(int n, const T *x, const T *grid_v, const Grid &g, T dx, T inv_dx,
 
 ORC_INSTANTIATE(float, f32)
 ORC_INSTANTIATE(double, f64)
+
+
+// ------------------------------------------------------------------ renderer C ABI (FP32)
+ORC_EXPORT void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  orc::rnd::philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+ORC_EXPORT void orc_uniforms(uint32_t seed, uint32_t pixel, uint32_t sample, int n, float *out) {
+  orc::rnd::Stream st;
+  st.seed = seed;
+  st.pixel = pixel;
+  st.sample = sample;
+  for (int i = 0; i < n; ++i) out[i] = st.uniform();
+}
+ORC_EXPORT void orc_sincos(int n, const float *phi, float *s, float *c) {
+  for (int i = 0; i < n; ++i) orc::rnd::sincos_taylor(phi[i], s[i], c[i]);
+}
+// particle_sdf (integrator.cu:1480-1525, 1463-1478, 1438-1461, 1908-1933): volume must be
+// filled with (1 << 33) - 1 by the caller
+ORC_EXPORT void orc_particle_sdf(int64_t *volume, const float *x, const int *color, const float *bbox_min,
+                                 int bake, const int *grid, float inv_dx, float *sdf, float *rgb, float *sdf_tmp,
+                                 int n) {
+  using orc::V3;
+  const int gx = grid[0], gy = grid[1], gz = grid[2], w = 2 * bake;
+  V3<float> lo{bbox_min[0], bbox_min[1], bbox_min[2]};
+  for (int p = 0; p < n; ++p)
+    for (int i = -bake; i < w - bake; ++i)
+      for (int j = -bake; j < w - bake; ++j)
+        for (int k = -bake; k < w - bake; ++k) {
+          V3<float> xs = (orc::load_v(x + 3 * p) - lo) * inv_dx;
+          int ix = (int)std::floor(xs.x) + i, iy = (int)std::floor(xs.y) + j, iz = (int)std::floor(xs.z) + k;
+          if (ix < 0 || iy < 0 || iz < 0 || ix >= gx || iy >= gy || iz >= gz) continue;
+          V3<float> dv{(float)ix - xs.x, (float)iy - xs.y, (float)iz - xs.z};
+          float dist = std::sqrt(orc::dot(dv, dv) + 0.f);
+          dist = std::min(std::max(0.0f, 51.0f * dist), 255.0f);
+          int64_t out = ((int64_t)dist << 24) + (int64_t)color[p];
+          int64_t &cell = volume[((int64_t)ix * gy + iy) * gz + iz];
+          if (out < cell) cell = out;
+        }
+  const int64_t dim = (int64_t)gx * gy * gz;
+  for (int64_t t = 0; t < dim; ++t) {
+    int64_t c = volume[t];
+    float zc = (float)((double)(c & 255) / 255.);
+    c >>= 8;
+    float yc = (float)((double)(c & 255) / 255.);
+    c >>= 8;
+    float xc = (float)((double)(c & 255) / 255.);
+    c >>= 8;
+    rgb[3 * t] = xc;
+    rgb[3 * t + 1] = yc;
+    rgb[3 * t + 2] = zc;
+    sdf[t] = (float)((double)(c & 255) / 255.);
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    const float *in = pass == 0 ? sdf : sdf_tmp;
+    float *out = pass == 0 ? sdf_tmp : sdf;
+    for (int x0 = 0; x0 < gx; ++x0)
+      for (int y0 = 0; y0 < gy; ++y0)
+        for (int z0 = 0; z0 < gz; ++z0) {
+          int64_t t = ((int64_t)x0 * gy + y0) * gz + z0;
+          if (x0 >= 1 && y0 >= 1 && z0 >= 1 && x0 < gx - 1 && y0 < gy - 1 && z0 < gz - 1) {
+            float s = 0.f;
+            for (int i = -1; i < 2; ++i)
+              for (int j = -1; j < 2; ++j)
+                for (int k = -1; k < 2; ++k) s += in[((int64_t)(x0 + i) * gy + (y0 + j)) * gz + (z0 + k)];
+            out[t] = (float)((double)s / 27.);
+          } else {
+            out[t] = 1.f;
+          }
+        }
+  }
+}
+// render (integrator.cu:1269-1436, 1863-1906) for the pixels [p_begin, p_end) of an
+// H x W image (p = row * W + column)
+ORC_EXPORT void orc_render(const float *sdf, const float *box_min, const float *box_max, const float *rgb,
+                           const float *bpos, const float *brot, const float *tfsr, const float *args,
+                           const float *cam_rot, const float *cam_pos, const float *cam_K, float *color_out,
+                           float *depth_out, float thr, const int *grid, int nb, int shape, int H, int W,
+                           int max_depth, int spp, float ground, int seed, const float *light, int p_begin,
+                           int p_end) {
+  using orc::V3;
+  orc::rdr::Scene S;
+  S.V.sdf = sdf;
+  S.V.color = rgb;
+  S.V.lo = orc::load_v(box_min);
+  S.V.hi = orc::load_v(box_max);
+  S.V.d[0] = grid[0];
+  S.V.d[1] = grid[1];
+  S.V.d[2] = grid[2];
+  S.thr = thr;
+  S.ground = ground;
+  S.nb = nb;
+  S.shape = shape != 0;
+  S.pos = bpos;
+  S.rot = brot;
+  S.tfsr = tfsr;
+  S.args = args;
+  V3<float> cp = orc::load_v(cam_pos), li = orc::load_v(light);
+  for (int p = p_begin; p < p_end; ++p) {
+    int u = p % W, v = p / W;
+    V3<float> acc{0.f, 0.f, 0.f};
+    float dacc = 0.f;
+    for (int s = 0; s < spp; ++s) {
+      orc::rnd::Stream rs;
+      rs.seed = (uint32_t)seed;
+      rs.pixel = (uint32_t)p;
+      rs.sample = (uint32_t)s;
+      V3<float> o;
+      float first;
+      orc::rdr::trace(S, cam_K, cam_rot, cp, li, max_depth, u, v, rs, o, first);
+      acc += o / (float)spp;
+      dacc += first / (float)spp;
+    }
+    float uu = (float)(1.0 * u / W), vv = (float)(1.0 * v / H);
+    float rad = std::sqrt((uu - 0.5f) * (uu - 0.5f) + (vv - 0.5f) * (vv - 0.5f));
+    float darken = (float)(1.0 - (double)(0.9f * std::max(rad - 0.f, 0.f)));
+    V3<float> c = acc * darken * 1.5f;
+    color_out[3 * p] = std::sqrt(c.x);
+    color_out[3 * p + 1] = std::sqrt(c.y);
+    color_out[3 * p + 2] = std::sqrt(c.z);
+    depth_out[p] = dacc;
+  }
+}
 
 ORC_EXPORT void orc_set_svd_mode(int mode) { orc::g_svd_mode = mode; }
 ORC_EXPORT int orc_get_svd_mode(void) { return orc::g_svd_mode; }

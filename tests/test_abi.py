@@ -12,7 +12,8 @@ HEADER = os.path.join(ROOT, "include", "dexdeform_mpm.h")
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|void\s*\*?|hipStream_t)\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt,
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|void\s*\*?|hipStream_t|ddm_texture_resources)\s*"
+                                 r"([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt,
                                  flags=re.M)))
 
 
@@ -46,18 +47,17 @@ def test_struct_layouts_match_header():
 
 
 # Every symbol the reference's ctypes layer binds (mpm/types.py:103-290) or calls
-# (print_memory_info, mpm/simulator.py:226), except the renderer's (render, particle_sdf,
-# create_volume, destroy_volume: the renderer is not part of this library).
+# (print_memory_info, mpm/simulator.py:226), the renderer's included.
 REFERENCE_CTYPES_SYMBOLS = [
     "cuda_alloc", "cuda_free", "cuda_upload", "cuda_upload_async", "cuda_download", "cuda_download_async",
     "cuda_copy", "cuda_copy2d", "cuda_zero", "cuda_zero_async", "cuda_stream_create", "cuda_stream_destroy",
     "cuda_stream_sync", "compute_grid_lower", "compute_svd", "compute_svd_grad", "p2g", "p2g_grad", "grid_op_v2",
-    "grid_op_v2_grad", "g2p", "g2p_grad", "compute_dist", "particle2mass", "print_memory_info"]
+    "grid_op_v2_grad", "g2p", "g2p_grad", "compute_dist", "particle2mass", "print_memory_info", "render",
+    "particle_sdf", "create_volume", "destroy_volume"]
 
 
 def test_library_exports_every_symbol_the_reference_binds():
-    """The reference's mpm/types.py can load this library: each non-renderer symbol it
-    binds is exported (checked with a plain ctypes.CDLL, as the reference loads it) and
+    """The reference's mpm/types.py can load this library: each symbol it binds is exported (checked with a plain ctypes.CDLL, as the reference loads it) and
     declared in the header."""
     import ctypes
     from dexdeform_amd import _lib
