@@ -15,9 +15,13 @@
 // is small; per-segment partial results are combined in a second, deterministic pass
 // (fixed segment order), so results do not depend on scheduling.
 //
-// The online log-sum-exp keeps a running maximum m and sum s per x point; a group of 8
-// pairs is reduced at a time (one rescale exp per group, one exp per pair). It is VALU
-// and transcendental bound: no MFMA (the p = 1 cost needs a square root per pair).
+// The online log-sum-exp keeps a running maximum m and sum s per x point in base 2 (the
+// weights are pre-scaled by log2 e when a tile is staged, so each pair costs one v_exp_f32
+// and no multiply); a group of 8 pairs is reduced at a time (one rescale exp per group).
+// The p = 1 distance is the hardware square root (v_sqrt_f32, 1 ulp) and |x - y|^2 and the
+// exponent are fused multiply-adds: about 10 VALU instructions per pair, 2 of them
+// transcendental. It is VALU and transcendental bound: no MFMA (the p = 1 cost needs a
+// square root per pair).
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
@@ -32,26 +36,35 @@ constexpr int kR = 2;        // x points per lane
 constexpr int kTile = 256;   // y points per LDS tile
 constexpr int kGroup = 8;    // pairs reduced together in the online log-sum-exp
 
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kLn2 = 0.693147180559945309f;
+
 template <int P>
 __device__ __forceinline__ float cost(float r2) {
-  if (P == 1) return sqrtf(r2);
+  if (P == 1) return __builtin_amdgcn_sqrtf(r2);
   return 0.5f * r2;
 }
 
-// Stages y[j0 .. j0+kTile) (and its weight; missing points get weight -inf) into LDS.
+__device__ __forceinline__ float dist2(const float *p, float4 q) {
+  const float dx = p[0] - q.x, dy = p[1] - q.y, dz = p[2] - q.z;
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Stages y[j0 .. j0+kTile) (and its weight times `scale`; missing points get weight
+// `pad`) into LDS.
 __device__ __forceinline__ void load_tile(float4 *t, const float *__restrict__ y, const float *__restrict__ w,
-                                          int j0, int j_end, float pad) {
+                                          int j0, int j_end, float pad, float scale = 1.f) {
   const int j = j0 + (int)threadIdx.x;
   float4 v = make_float4(0.f, 0.f, 0.f, pad);
-  if (j < j_end) v = make_float4(y[3 * j], y[3 * j + 1], y[3 * j + 2], w ? w[j] : 0.f);
+  if (j < j_end) v = make_float4(y[3 * j], y[3 * j + 1], y[3 * j + 2], w ? w[j] * scale : 0.f);
   t[threadIdx.x] = v;
 }
 
 // Partial log-sum-exp of one j segment: part[seg][i] = (m, s) with
-// sum_{j in seg} exp(h_j - C_ij / eps) = s * exp(m).
+// sum_{j in seg} exp(h_j - C_ij / eps) = s * 2^m. inv_eps2 = log2(e) / eps.
 template <int P>
 __global__ __launch_bounds__(kWG) void k_softmin_part(const float *__restrict__ x, int n, const float *__restrict__ y,
-                                                      const float *__restrict__ h, int m, int seg_len, float inv_eps,
+                                                      const float *__restrict__ h, int m, int seg_len, float inv_eps2,
                                                       float2 *__restrict__ part) {
   __shared__ float4 tile[kTile];
   const int seg = blockIdx.y;
@@ -70,7 +83,7 @@ __global__ __launch_bounds__(kWG) void k_softmin_part(const float *__restrict__ 
   }
   for (int j0 = j_begin; j0 < j_end; j0 += kTile) {
     __syncthreads();
-    load_tile(tile, y, h, j0, j_end, -INFINITY);
+    load_tile(tile, y, h, j0, j_end, -INFINITY, kLog2e);
     __syncthreads();
     const int cnt = min(kTile, j_end - j0);
     for (int k0 = 0; k0 < cnt; k0 += kGroup) {
@@ -79,10 +92,7 @@ __global__ __launch_bounds__(kWG) void k_softmin_part(const float *__restrict__ 
       for (int g = 0; g < kGroup; ++g) {
         const float4 q = tile[k0 + g];  // entries past cnt hold weight -inf
 #pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          const float dx = px[r][0] - q.x, dy = px[r][1] - q.y, dz = px[r][2] - q.z;
-          t[r][g] = q.w - cost<P>(dx * dx + dy * dy + dz * dz) * inv_eps;
-        }
+        for (int r = 0; r < kR; ++r) t[r][g] = fmaf(-cost<P>(dist2(px[r], q)), inv_eps2, q.w);
       }
 #pragma unroll
       for (int r = 0; r < kR; ++r) {
@@ -90,12 +100,12 @@ __global__ __launch_bounds__(kWG) void k_softmin_part(const float *__restrict__ 
 #pragma unroll
         for (int g = 1; g < kGroup; ++g) gm = fmaxf(gm, t[r][g]);
         if (gm > mx[r]) {
-          sm[r] *= __expf(mx[r] - gm);
+          sm[r] *= __builtin_amdgcn_exp2f(mx[r] - gm);
           mx[r] = gm;
         }
         float acc = 0.f;
 #pragma unroll
-        for (int g = 0; g < kGroup; ++g) acc += __expf(t[r][g] - mx[r]);
+        for (int g = 0; g < kGroup; ++g) acc += __builtin_amdgcn_exp2f(t[r][g] - mx[r]);
         sm[r] += acc;
       }
     }
@@ -107,7 +117,7 @@ __global__ __launch_bounds__(kWG) void k_softmin_part(const float *__restrict__ 
   }
 }
 
-// out_i = -eps (M + log S) over the segments in order
+// out_i = -eps ln2 (M + log2 S) over the segments in order
 __global__ void k_softmin_combine(const float2 *__restrict__ part, int n, int segs, float eps,
                                   float *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,18 +127,19 @@ __global__ void k_softmin_combine(const float2 *__restrict__ part, int n, int se
   float S = 0.f;
   for (int s = 0; s < segs; ++s) {
     const float2 v = part[(size_t)s * n + i];
-    S += v.y * __expf(v.x - M);
+    S += v.y * exp2f(v.x - M);
   }
-  out[i] = -eps * (M + __logf(S));
+  out[i] = -eps * kLn2 * (M + log2f(S));
 }
 
 // Partial plan-weighted cost gradients of one j segment:
-// part[seg][i] = sum_{j in seg} exp(a_i + b_j - C_ij / eps) dC/dx(x_i, y_j)
+// part[seg][i] = sum_{j in seg} exp(a_i + b_j - C_ij / eps) dC/dx(x_i, y_j), evaluated as
+// 2^((a_i + b_j) log2 e - C_ij log2 e / eps); inv_eps2 = log2(e) / eps.
 template <int P>
 __global__ __launch_bounds__(kWG) void k_plan_grad_part(const float *__restrict__ x, const float *__restrict__ a,
                                                         int n, const float *__restrict__ y,
                                                         const float *__restrict__ b, int m, int seg_len,
-                                                        float inv_eps, float4 *__restrict__ part) {
+                                                        float inv_eps2, float4 *__restrict__ part) {
   __shared__ float4 tile[kTile];
   const int seg = blockIdx.y;
   const int j_begin = seg * seg_len, j_end = min(m, j_begin + seg_len);
@@ -140,12 +151,12 @@ __global__ __launch_bounds__(kWG) void k_plan_grad_part(const float *__restrict_
     px[r][0] = x[3 * ic];
     px[r][1] = x[3 * ic + 1];
     px[r][2] = x[3 * ic + 2];
-    pa[r] = a[ic];
+    pa[r] = a[ic] * kLog2e;
     acc[r][0] = acc[r][1] = acc[r][2] = 0.f;
   }
   for (int j0 = j_begin; j0 < j_end; j0 += kTile) {
     __syncthreads();
-    load_tile(tile, y, b, j0, j_end, -INFINITY);
+    load_tile(tile, y, b, j0, j_end, -INFINITY, kLog2e);
     __syncthreads();
     const int cnt = min(kTile, j_end - j0);
     for (int k = 0; k < cnt; ++k) {
@@ -153,16 +164,16 @@ __global__ __launch_bounds__(kWG) void k_plan_grad_part(const float *__restrict_
 #pragma unroll
       for (int r = 0; r < kR; ++r) {
         const float dx = px[r][0] - q.x, dy = px[r][1] - q.y, dz = px[r][2] - q.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         float wgt, gscale;
         if (P == 1) {
           // d|x - y|/dx = (x - y) / |x - y|; 0 at coincident points
           const bool pos = r2 > 0.f;
-          const float inv_d = pos ? rsqrtf(r2) : 0.f;
-          wgt = __expf(pa[r] + q.w - r2 * inv_d * inv_eps);
+          const float inv_d = pos ? __builtin_amdgcn_rsqf(r2) : 0.f;
+          wgt = __builtin_amdgcn_exp2f(fmaf(-r2 * inv_d, inv_eps2, pa[r] + q.w));
           gscale = wgt * inv_d;
         } else {
-          wgt = __expf(pa[r] + q.w - 0.5f * r2 * inv_eps);
+          wgt = __builtin_amdgcn_exp2f(fmaf(-0.5f * r2, inv_eps2, pa[r] + q.w));
           gscale = wgt;
         }
         acc[r][0] += gscale * dx;
@@ -221,8 +232,7 @@ __global__ __launch_bounds__(kWG) void k_nearest_part(const float *__restrict__ 
       const float4 q = tile[k];
 #pragma unroll
       for (int r = 0; r < kR; ++r) {
-        const float dx = px[r][0] - q.x, dy = px[r][1] - q.y, dz = px[r][2] - q.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = dist2(px[r], q);
         if (r2 < best[r]) {
           best[r] = r2;
           arg[r] = j0 + k;
@@ -285,8 +295,9 @@ DDM_API int ddm_softmin(const float *x, int n, const float *y, const float *h, i
   const int segs = (m + seg_len - 1) / seg_len;
   float2 *part = static_cast<float2 *>(workspace);
   dim3 grid(blocks_x(n), segs);
-  if (p == 1) k_softmin_part<1><<<grid, kWG, 0, stream>>>(x, n, y, h, m, seg_len, 1.f / eps, part);
-  else k_softmin_part<2><<<grid, kWG, 0, stream>>>(x, n, y, h, m, seg_len, 1.f / eps, part);
+  const float inv_eps2 = kLog2e / eps;
+  if (p == 1) k_softmin_part<1><<<grid, kWG, 0, stream>>>(x, n, y, h, m, seg_len, inv_eps2, part);
+  else k_softmin_part<2><<<grid, kWG, 0, stream>>>(x, n, y, h, m, seg_len, inv_eps2, part);
   k_softmin_combine<<<(n + 255) / 256, 256, 0, stream>>>(part, n, segs, eps, out);
   return (int)hipGetLastError();
 }
@@ -301,8 +312,9 @@ DDM_API int ddm_plan_grad(const float *x, const float *a, int n, const float *y,
   const int segs = (m + seg_len - 1) / seg_len;
   float4 *part = static_cast<float4 *>(workspace);
   dim3 grid(blocks_x(n), segs);
-  if (p == 1) k_plan_grad_part<1><<<grid, kWG, 0, stream>>>(x, a, n, y, b, m, seg_len, 1.f / eps, part);
-  else k_plan_grad_part<2><<<grid, kWG, 0, stream>>>(x, a, n, y, b, m, seg_len, 1.f / eps, part);
+  const float inv_eps2 = kLog2e / eps;
+  if (p == 1) k_plan_grad_part<1><<<grid, kWG, 0, stream>>>(x, a, n, y, b, m, seg_len, inv_eps2, part);
+  else k_plan_grad_part<2><<<grid, kWG, 0, stream>>>(x, a, n, y, b, m, seg_len, inv_eps2, part);
   k_grad_combine<<<(n + 255) / 256, 256, 0, stream>>>(part, n, segs, grad);
   return (int)hipGetLastError();
 }
