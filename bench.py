@@ -552,6 +552,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-score", action="store_true", help="skip the shape-score (Sinkhorn, Chamfer) leg")
     ap.add_argument("--no-render", action="store_true", help="skip the step + render leg")
+    ap.add_argument("--only", default=None, metavar="LEGS",
+                    help="diagnostic: run only these legs (comma list of score, render, trajopt) and print "
+                         "one JSON line of them (for profiling a leg)")
     ap.add_argument("--no-tape", action="store_true",
                     help="adjoint recomputes P2G and the grid update (as the reference does) instead of "
                          "reading the forward pass's tape")
@@ -578,6 +581,10 @@ def main():
 
     import torch
     dev = torch.device("cuda:0")
+    if args.only:  # diagnostic: only the listed legs, one JSON line of them
+        legs = {"score": score_benchmark, "render": render_benchmark, "trajopt": trajectory_optimisation}
+        print(json.dumps({name: legs[name](dev) for name in args.only.split(",")}), flush=True)
+        return
     if args.named_only:  # diagnostic: only the named single-scene rollouts, one JSON line of them
         print(json.dumps({"single_scene_rollouts": named_rollouts(dev, max(1, args.steps),
                                                                   only=args.named_only.split(","))}), flush=True)

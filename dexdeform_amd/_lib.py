@@ -69,8 +69,9 @@ assert ctypes.sizeof(DdmDet) == 80
 
 
 class DdmTextureResources(ctypes.Structure):
-    """(hipArray_t, hipTextureObject_t) returned by create_volume (types.py's texture_resources)."""
-    _fields_ = [("array", c_void_p), ("texture", c_void_p)]
+    """(device volume, texture handle) returned by create_volume (the layout of types.py's
+    texture_resources; the handle is 0 on gfx950, which has no texture objects)."""
+    _fields_ = [("volume", c_void_p), ("texture", ctypes.c_ulonglong)]
 
 
 class DdmBatch(ctypes.Structure):

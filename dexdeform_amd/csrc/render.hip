@@ -563,47 +563,24 @@ DDM_API int render(float *sdf_volume, float *box_min, float *box_max, float *col
   return (int)hipGetLastError();
 }
 
-// create_volume / destroy_volume (integrator.cu:2074-2124): a float4 volume (x fastest,
-// then y, then z; host memory) copied into a 3-D array with a linearly filtered, clamped,
-// unnormalised-coordinate texture object. Bound by mpm/types.py:121-124; nothing in the
-// reference samples it. Allocates (as the reference's does); on failure both handles are
-// null.
+// create_volume / destroy_volume (integrator.cu:2074-2124, bound by mpm/types.py:121-124;
+// nothing in the reference samples the volume). The reference uploads a float4 volume
+// into a 3-D CUDA array with a linearly filtered texture object; gfx950 has no texture
+// objects (hipMalloc3DArray and hipCreateTextureObject return hipErrorNotSupported), so
+// the volume is a plain device copy (float4 per voxel, x fastest, then y, then z) and the
+// texture handle is 0. Allocates, as the reference's does; on failure the handles are null.
 DDM_API ddm_texture_resources create_volume(float *data, int x, int y, int z) {
-  ddm_texture_resources r{nullptr, nullptr};
+  ddm_texture_resources r{nullptr, 0};
   if (!data || x <= 0 || y <= 0 || z <= 0) return r;
-  hipChannelFormatDesc ch = hipCreateChannelDesc(32, 32, 32, 32, hipChannelFormatKindFloat);
-  hipArray_t arr = nullptr;
-  if (hipMalloc3DArray(&arr, &ch, make_hipExtent(x, y, z), 0) != hipSuccess) return r;
-  hipMemcpy3DParms cp = {};
-  cp.srcPtr = make_hipPitchedPtr(data, (size_t)x * 16, x, y);
-  cp.dstArray = arr;
-  cp.extent = make_hipExtent(x, y, z);
-  cp.kind = hipMemcpyHostToDevice;
-  hipResourceDesc res = {};
-  res.resType = hipResourceTypeArray;
-  res.res.array.array = arr;
-  hipTextureDesc td = {};
-  td.addressMode[0] = td.addressMode[1] = td.addressMode[2] = hipAddressModeClamp;
-  td.filterMode = hipFilterModeLinear;
-  td.readMode = hipReadModeElementType;
-  td.normalizedCoords = 0;
-  hipTextureObject_t tex = nullptr;
-  if (hipMemcpy3D(&cp) != hipSuccess || hipCreateTextureObject(&tex, &res, &td, nullptr) != hipSuccess) {
-    (void)hipFreeArray(arr);
+  const size_t bytes = (size_t)x * (size_t)y * (size_t)z * 4 * sizeof(float);
+  void *d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) return r;
+  if (hipMemcpy(d, data, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
     return r;
   }
-  r.array = arr;
-  r.texture = tex;
+  r.volume = d;
   return r;
 }
 
-DDM_API int destroy_volume(ddm_texture_resources tex) {
-  int e = 0;
-  if (tex.texture) e = (int)hipDestroyTextureObject(tex.texture);
-  if (tex.array) {
-    const int e2 = (int)hipFreeArray(tex.array);
-    if (!e) e = e2;
-  }
-  return e;
-}
-
+DDM_API int destroy_volume(ddm_texture_resources tex) { return tex.volume ? (int)hipFree(tex.volume) : 0; }

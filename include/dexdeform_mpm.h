@@ -355,13 +355,15 @@ int render(float *sdf_volume, float *box_min, float *box_max, float *color_volum
            float ground_height, int seed, float *light_direction, hipStream_t stream);
 
 /* create_volume / destroy_volume (integrator.cu:2074-2124, bound by mpm/types.py:121-124):
- * a float4 volume in host memory (x fastest, then y, z) copied into a 3-D array with a
- * linearly filtered, clamped texture object. They allocate and free, as the reference's
- * do; nothing in the reference samples the texture. On failure create_volume returns
- * null handles; destroy_volume returns 0 or the first hipError_t. */
+ * the reference uploads a float4 host volume into a 3-D CUDA array with a linearly
+ * filtered texture object; nothing in the reference samples it. gfx950 has no texture
+ * objects (hipErrorNotSupported), so `volume` is a plain device copy (float4 per voxel,
+ * x fastest, then y, then z) and `texture` is 0. Both allocate/free, as the reference's
+ * do; on failure create_volume returns a null volume; destroy_volume returns 0 or the
+ * hipError_t of the free. */
 typedef struct {
-  hipArray_t array;
-  hipTextureObject_t texture;
+  void *volume;
+  unsigned long long texture;
 } ddm_texture_resources; /* 16 bytes: the layout of types.py's texture_resources */
 ddm_texture_resources create_volume(float *data, int x, int y, int z);
 int destroy_volume(ddm_texture_resources tex);

@@ -192,13 +192,17 @@ def test_dropin_renderer_on_a_simulator_state():
 
 
 def test_texture_volume_helpers():
-    """create_volume / destroy_volume (the reference's texture helpers): a float4 host volume
-    becomes a 3-D array with a texture object, and both are released."""
+    """create_volume / destroy_volume (the reference's texture helpers): the float4 host
+    volume is copied to the device (gfx950 has no texture objects: the handle is 0) and
+    released."""
     lib = _lib.load()
     data = np.arange(5 * 4 * 3 * 4, dtype=np.float32)
-    tex = lib.create_volume(data.ctypes.data, 5, 4, 3)
-    assert tex.array and tex.texture
-    assert lib.destroy_volume(tex) == 0
+    vol = lib.create_volume(data.ctypes.data, 5, 4, 3)
+    assert vol.volume and vol.texture == 0
+    back = np.zeros_like(data)
+    _lib.check(lib.cuda_download(back.ctypes.data, vol.volume, data.nbytes), "cuda_download")
+    np.testing.assert_array_equal(back, data)
+    assert lib.destroy_volume(vol) == 0
     bad = lib.create_volume(data.ctypes.data, 0, 4, 3)
-    assert not bad.array and not bad.texture
+    assert not bad.volume
     torch.cuda.synchronize()
