@@ -86,6 +86,7 @@ class DdmBatch(ctypes.Structure):
 
 DDM_GRAD_OVERWRITE = 1
 DDM_DETERMINISTIC = 2
+DDM_TAPE_COMPLETE = 4
 
 
 class DdmState(ctypes.Structure):

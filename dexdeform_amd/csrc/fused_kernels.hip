@@ -632,9 +632,14 @@ static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
   k_restore<<<dim3(kRestoreWGs, b->n_scenes), kWG, 0, stream>>>(*b);
-  if (b->options & DDM_DETERMINISTIC) launch_det_p2g<kRecomputeUntaped>(b, cur, next, stream);
-  else k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
-  k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
+  // With a complete tape every scene restores its whole substep and both recompute kernels
+  // would exit at once; the deterministic recompute P2G also rebuilds the per-cell particle
+  // order that the sorted adjoint gathers read, so it always runs.
+  if (!(b->options & DDM_TAPE_COMPLETE) || (b->options & DDM_DETERMINISTIC)) {
+    if (b->options & DDM_DETERMINISTIC) launch_det_p2g<kRecomputeUntaped>(b, cur, next, stream);
+    else k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+    k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
+  }
   mark(ev, 1, stream);
   launch_adjoint_tail(b, cur, next, cur_grad, next_grad, stream, ev, 2);
   return (int)hipGetLastError();

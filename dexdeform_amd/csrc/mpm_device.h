@@ -262,13 +262,16 @@ DDM_HD f3 prim_grad_adj(const float *prim, const float *args, f3 p, f3 g_out, in
 // (8 sweeps), column sort by norm, Givens QR. The float-rounded constants of the
 // reference (svd.h:17-20) are kept so that the same rotations are chosen.
 namespace svd {
-// 1/sqrt(x) in FP64: the hardware estimate refined by three Newton steps (within a few
-// FP64 ulps of the correctly rounded value; the factors are returned in FP32, where this
-// difference from the reference's sqrt and division does not show)
+// The SVD's FP64 arithmetic contracts multiply-adds into FMAs (as nvcc does by default for
+// the reference's svd.h); the factors are returned in FP32, where the rounding differences
+// this makes, and those of rsq64 and of the Newton reciprocal in jacobi, do not show
+// beyond FP32 rounding (the parity tests against the oracle are unchanged).
+// 1/sqrt(x) in FP64: the hardware estimate (about 23 bits) refined by two Newton steps
+// (about 46 bits; the reference divides by a correctly rounded sqrt)
 DDM_HD double rsq64(double x) {
+#pragma clang fp contract(fast)
   double y = __builtin_amdgcn_rsq(x);
   const double h = 0.5 * x;
-  y = y * (1.5 - h * (y * y));
   y = y * (1.5 - h * (y * y));
   y = y * (1.5 - h * (y * y));
   return y;
@@ -282,13 +285,18 @@ constexpr double kEps = 1e-6;
 // then rotate the storage so the next call targets the next index pair (svd.h:126-186)
 template <int X, int Y, int Z>
 DDM_HD void jacobi(double &s11, double &s21, double &s22, double &s31, double &s32, double &s33, double q[4]) {
+#pragma clang fp contract(fast)
   double ch = 2.0 * (s11 - s22), sh = s21;
   bool b = kGamma * sh * sh < ch * ch;
   double w = rsq64(ch * ch + sh * sh);
   ch = b ? w * ch : kCstar;
   sh = b ? w * sh : kSstar;
   double scale = ch * ch + sh * sh;
-  const double inv = 1.0 / scale;  // one division for both quotients
+  // 1 / scale (one reciprocal for both quotients, svd.h:133-135) by two Newton steps from 1:
+  // scale is 1 up to rounding (normalised (ch, sh)) or 1 - 5.7e-8 (the float-rounded
+  // cos/sin(pi/8)), so two steps reach FP64 rounding without a division
+  double inv = 2.0 - scale;
+  inv = inv * (2.0 - scale * inv);
   double a = (ch * ch - sh * sh) * inv;
   double bb = (2.0 * sh * ch) * inv;
   double n11 = a * (a * s11 + bb * s21) + bb * (a * s21 + bb * s22);
@@ -318,6 +326,7 @@ DDM_HD void cnswap(bool c, double &x, double &y) {
   y = c ? t : y;
 }
 DDM_HD void qr_rot(double a1, double a2, double &ch, double &sh) {  // svd.h:248-262
+#pragma clang fp contract(fast)
   double r2 = a1 * a1 + a2 * a2;
   double rho = r2 * rsq64(r2);  // r2 / sqrt(r2); NaN for r2 = 0 as there
   sh = rho > kEps ? a2 : 0.0;
@@ -331,6 +340,7 @@ DDM_HD void qr_rot(double a1, double a2, double &ch, double &sh) {  // svd.h:248
 
 // F = U diag(sig) V^T ; F given in FP32, factors returned in FP32 (svd.h:390-413)
 DDM_HD void svd3(const m33 &F, m33 &U, f3 &sig, m33 &V) {
+#pragma clang fp contract(fast)
   using namespace svd;
   double a[9];
 #pragma unroll

@@ -357,11 +357,31 @@ class Engine:
                    "ddm_substep_backward")
 
     # ------------------------------------------------------------------ taped adjoint
-    def tape_capacities(self) -> List[int]:
-        """Default snapshot size per scene (blocks): every block of a small grid, else
-        particles / 32 + 64 (a substep that lists more is recomputed by its adjoint)."""
-        blocks = [(int(s.grid_dim[0]) // 4) * (int(s.grid_dim[1]) // 4) * (int(s.grid_dim[2]) // 4) for s in self.scenes]
+    # a snapshot block holds (m, mv) and v of 64 nodes: 2 KiB
+    TAPE_BLOCK_BYTES = 2048
+    # whole-grid snapshots for every scene when they take at most this much per substep
+    TAPE_COMPLETE_BYTES = 64 << 20
+
+    def grid_blocks(self) -> List[int]:
+        return [(int(s.grid_dim[0]) // 4) * (int(s.grid_dim[1]) // 4) * (int(s.grid_dim[2]) // 4) for s in self.scenes]
+
+    def tape_capacities(self, whole_grid: Optional[bool] = None) -> List[int]:
+        """Snapshot size per scene (blocks). whole_grid=True: every block of each grid (the
+        adjoint then never recomputes, see tape_complete); False: the whole grid of a small
+        one, else particles / 32 + 64 blocks (a substep that lists more is recomputed by its
+        adjoint); None (default): whole grids when all scenes' whole grids take at most
+        TAPE_COMPLETE_BYTES per substep."""
+        blocks = self.grid_blocks()
+        if whole_grid is None:
+            whole_grid = sum(blocks) * self.TAPE_BLOCK_BYTES <= self.TAPE_COMPLETE_BYTES
+        if whole_grid:
+            return blocks
         return [min(b, int(c) // 32 + 64) for b, c in zip(blocks, self.capacity)]
+
+    def tape_complete(self, caps) -> bool:
+        """Whether snapshots of these capacities can hold every block each scene's grid has
+        (no scene's substep can overflow: the taped adjoint skips its recompute kernels)."""
+        return len(caps) == len(self.scenes) and all(int(c) >= b for c, b in zip(caps, self.grid_blocks()))
 
     def taped_table(self, temps, tape_mmv, tape_v, tape_list, tape_status, caps) -> torch.Tensor:
         """Scene table of one taped substep: each scene's F_tmp/U/V/sigma temporaries in
@@ -408,11 +428,15 @@ class Engine:
         _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                       _lib.stream_ptr(stream)), "ddm_substep_forward_taped")
 
-    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, stream=None, overwrite=False):
-        """Adjoint substep from what forward_taped recorded with the same scene table."""
+    def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, stream=None, overwrite=False,
+                       complete=False):
+        """Adjoint substep from what forward_taped recorded with the same scene table.
+        complete=True: every scene's snapshot holds its whole grid (tape_complete), so the
+        recompute kernels for overflowing scenes are not launched."""
         if not self.with_grad:
             raise RuntimeError("engine was created with with_grad=False")
-        b = self._batch_with(scene_tab, _lib.DDM_GRAD_OVERWRITE if overwrite else 0)
+        opts = (_lib.DDM_GRAD_OVERWRITE if overwrite else 0) | (_lib.DDM_TAPE_COMPLETE if complete else 0)
+        b = self._batch_with(scene_tab, opts)
         _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                        gcur_tab.data_ptr(), gnxt_tab.data_ptr(),
                                                        _lib.stream_ptr(stream)), "ddm_substep_backward_taped")

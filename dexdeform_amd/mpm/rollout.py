@@ -63,6 +63,7 @@ class _Tapes:
         dev = engine.device
         P = engine.n_particles_total
         self.caps = [int(c) for c in caps]
+        self.complete = engine.tape_complete(self.caps)  # no substep can overflow its snapshot
         nblk = sum(self.caps)
         self.temps = torch.empty(slots, 30 * P, dtype=torch.float32, device=dev)
         self.grid_mmv = torch.empty(slots, nblk * 256, dtype=torch.float32, device=dev)
@@ -127,6 +128,7 @@ class Rollout:
             else:
                 caps = [int(tape_blocks)] * len(engine.scenes)
             self.tape = _Tapes(engine, self.K, caps)
+        self.tape_complete = self.tape is not None and self.tape.complete
 
     # ------------------------------------------------------------------ inputs / outputs
     def set_initial(self, x, v=None, F=None, C=None):
@@ -200,7 +202,7 @@ class Rollout:
                         self._gview(nxt_slot, t + 1))
                 # the adjoint substep writes (not adds to) the state-t gradient: no clearing
                 if self.tape is not None:
-                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], overwrite=True)
+                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], overwrite=True, complete=self.tape_complete)
                 else:
                     eng.backward_tables(*tabs, overwrite=True)
         self._g0 = self.g[cur_slot]

@@ -149,13 +149,15 @@ class GradModel:
         if self._tapes is None:
             from .rollout import _Tapes
             eng, sim = self.sim.engine, self.sim
-            caps = eng.tape_capacities()
             P = eng.n_particles_total
             slots = sim.max_steps - 1
-            need = slots * (30 * P * 4 + sum(caps) * (2 * 256 * 4 + 4) + 8)
+            need = lambda caps: slots * (30 * P * 4 + sum(caps) * (2 * 256 * 4 + 4) + 8)  # noqa: E731
+            budget = min(16 << 30, torch.cuda.get_device_properties(sim.device).total_memory // 4)
+            caps = eng.tape_capacities()
+            if need(caps) > budget:  # whole-grid snapshots for every substep do not fit: bounded ones
+                caps = eng.tape_capacities(whole_grid=False)
             if self.tape is None:
-                total = torch.cuda.get_device_properties(sim.device).total_memory
-                self.tape = need <= min(16 << 30, total // 4)
+                self.tape = need(caps) <= budget
                 if not self.tape:
                     return None
             self._tapes = _Tapes(eng, slots, caps)
@@ -232,7 +234,8 @@ class GradModel:
                     eng.backward(cur.buf, nxt.buf, cur.grad, nxt.grad, overwrite=True)
                 else:
                     eng.backward_taped(eng.table_of(cur.buf), eng.table_of(nxt.buf), eng.table_of(cur.grad),
-                                       eng.table_of(nxt.grad), tapes.scene_tabs[i], overwrite=True)
+                                       eng.table_of(nxt.grad), tapes.scene_tabs[i], overwrite=True,
+                                       complete=tapes.complete)
             # the pose gradients of states f+1 .. f+S are complete once the step's adjoint
             # substeps have run (later adjoint substeps touch state f and earlier only)
             torch._foreach_copy_(dst, src)

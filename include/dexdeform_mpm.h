@@ -158,6 +158,13 @@ int particle2mass(float *particle_x, float *particle_m, int *grid_lower, const i
 #define DDM_MAX_BODIES 64
 #define DDM_GRAD_OVERWRITE 1  /* ddm_batch.options */
 #define DDM_DETERMINISTIC 2   /* ddm_batch.options; needs ddm_batch.det */
+#define DDM_TAPE_COMPLETE 4   /* ddm_batch.options, ddm_substep_backward_taped only: the caller
+                                 guarantees that every launched scene keeps a snapshot
+                                 (tape_mmv set) whose tape_capacity is at least the number of
+                                 blocks of its grid, so no snapshot can overflow and the
+                                 recompute kernels of overflowing scenes are not launched
+                                 (ignored with DDM_DETERMINISTIC, whose adjoint reads the
+                                 particle order its recompute pass builds) */
 
 typedef struct ddm_scene {
   int32_t p_count;             /* particles of the scene (chunks define the work) */

@@ -2,9 +2,9 @@
 hand, model scale 1.5, fixed wrist, hand friction 0.9, 19 capsule/box geoms, quality 1 so
 40 substeps per environment step, ground friction 0.3, gravity (0, -2, 0)) on the GPU.
 
-* ``step`` (the forward-only path the planner uses) over two environment steps of seeded
+* ``step`` (the forward-only path the planner uses) over four environment steps of seeded
   joint-velocity actions against the FP32 oracle driven with the same geom poses:
-  x, v, F, C within 1e-4 relative after 80 substeps;
+  x, v, F, C within 1e-4 relative after 160 substeps;
 * ``get_state`` / ``set_state`` round trip, with the geom poses following from the wrist
   frame and joint angles;
 * GradModel over two environment steps, L2 loss to a seeded target, loss.backward():
@@ -54,9 +54,9 @@ def _sim(max_steps):
     return sim, x0
 
 
-def _actions(seed=6):
+def _actions(seed=6, steps=ENV_STEPS):
     rng = np.random.default_rng(seed)
-    return rng.uniform(-1.0, 1.0, size=(ENV_STEPS, 1, 26)).astype(np.float32)
+    return rng.uniform(-1.0, 1.0, size=(steps, 1, 26)).astype(np.float32)
 
 
 def _oracle(sim, dtype):
@@ -77,9 +77,12 @@ def _initial(sim, x0, dtype):
                    np.zeros((N, 3, 3), dtype), pos, rot)
 
 
+STEP_ENV_STEPS = 4
+
+
 def test_hand_step_matches_oracle_with_the_same_geom_poses():
     sim, x0 = _sim(max_steps=41)
-    acts = _actions()
+    acts = _actions(steps=STEP_ENV_STEPS)
     s0 = _initial(sim, x0, np.float32)  # particles and geom poses before the steps
     P, R = [sim.states[0].body_pos.download()], [sim.states[0].body_rot.download()]
     for a in acts:
@@ -95,7 +98,8 @@ def test_hand_step_matches_oracle_with_the_same_geom_poses():
     print(f"max particle displacement {moved:.3e}")
     assert moved > 1e-4, "the hand did not touch the particles"
     errs = {name: rel_err(g, r) for name, g, r in zip("xvFC", got[:4], (ref.x, ref.v, ref.F, ref.C))}
-    print(f"hand step x{ENV_STEPS} (80 substeps) vs oracle: " + ", ".join(f"{k} {v:.3e}" for k, v in errs.items()))
+    print(f"hand step x{STEP_ENV_STEPS} ({40 * STEP_ENV_STEPS} substeps) vs oracle: "
+          + ", ".join(f"{k} {v:.3e}" for k, v in errs.items()))
     assert max(errs.values()) <= 1e-4, errs
     # the hand state advanced with the steps
     assert np.allclose(got[-2], sim.base_pose[0].cpu().numpy())
