@@ -46,9 +46,11 @@ class HandKinematics:
 
     Joint j's frame is its chain predecessor's frame times the fixed frames between them
     times T_j = [axis_angle(axis_j * q_j) | pos_j]; each collision geom's frame is its
-    joint's frame times the geom's frame (capsules turned by pi/2 about x). The chains are
-    evaluated site by site in the order of ``site_computations``, as the reference does,
-    so the matrix products associate the same way."""
+    joint's frame times the geom's frame (capsules turned by pi/2 about x). The reference
+    walks ``site_computations`` site by site; the chains share their prefixes (the wrist
+    and palm joints open every finger's chain), so here the joints are grouped by depth in
+    the tree the chains describe and each depth is one batched product, with the same
+    association per joint ((parent frame @ fixed frames) @ T_j, left to right)."""
 
     def __init__(self, hand_cfg: dict, device="cuda:0", dtype=torch.float32):
         self.device = torch.device(device)
@@ -70,6 +72,48 @@ class HandKinematics:
             while steps and steps[-1][0] == "mat":
                 steps.pop()
             self.chains.append(steps)
+        # the tree: parent joint (-1: the wrist frame) and fixed frames before each joint,
+        # identical in every chain that passes through it; joints grouped by depth
+        parent, pre = [None] * self.n_joints, [None] * self.n_joints
+        for steps in self.chains:
+            p, mats = -1, []
+            for kind, v in steps:
+                if kind == "mat":
+                    mats.append(v)
+                    continue
+                if parent[v] is None:
+                    parent[v], pre[v] = p, mats
+                elif parent[v] != p or len(pre[v]) != len(mats) or any(
+                        not torch.equal(a, b) for a, b in zip(pre[v], mats)):
+                    raise ValueError(f"joint {v} is reached along different paths")
+                p, mats = v, []
+        missing = [j for j in range(self.n_joints) if parent[j] is None]
+        if missing:
+            raise ValueError(f"joints {missing} are in no chain")
+        depth = [0] * self.n_joints
+        for j in range(self.n_joints):  # chains list ancestors before descendants
+            d, k = 0, j
+            while parent[k] >= 0:
+                d, k = d + 1, parent[k]
+            depth[j] = d
+        if any(len(m) > 1 for m in pre):
+            raise ValueError("more than one fixed frame between two joints is not supported")
+        eye = torch.eye(4, dtype=dtype, device=self.device).expand(self.n_hands, 4, 4)
+        # per depth: (joints, slots of their parents in the frame stack, fixed frames
+        # [n_hands, n, 4, 4]); the stack holds the wrist frame at slot 0, then each depth's
+        # joints in order
+        self.levels = []
+        slot = {-1: 0}
+        for d in range(max(depth) + 1):
+            js = [j for j in range(self.n_joints) if depth[j] == d]
+            par = [slot[parent[j]] for j in js]
+            M = torch.stack([pre[j][0] if pre[j] else eye for j in js], 1)
+            self.levels.append((torch.as_tensor(js, dtype=torch.long, device=self.device),
+                                torch.as_tensor(par, dtype=torch.long, device=self.device), M))
+            first = len(slot)
+            for i, j in enumerate(js):
+                slot[j] = first + i
+        self.frame_slot = torch.as_tensor([slot[j] for j in range(self.n_joints)], dtype=torch.long, device=self.device)
         ji = hand_cfg["joint_info"]
         self.joint_pos = f(np.stack([np.asarray(p) for p, _ in ji])).reshape(self.n_joints, self.n_hands, 3)
         self.joint_pos = self.joint_pos.transpose(0, 1).contiguous()
@@ -120,16 +164,12 @@ class HandKinematics:
         T[..., :3, :3] = axis_angle_to_matrix(self.joint_axis.to(q.dtype)[None] * q[..., None])
         T[..., :3, 3] = self.joint_pos.to(q.dtype)
         T[..., 3, 3] = 1.0
-        frames = [None] * self.n_joints
-        for steps in self.chains:
-            base = base_pose
-            for kind, v in steps:
-                if kind == "mat":
-                    base = base @ v.to(q.dtype)
-                else:
-                    base = base @ T[..., v, :, :]
-                    frames[v] = base
-        J = torch.stack(frames, -3)
+        # frame stack: slot 0 the wrist, then the joints depth by depth
+        F = base_pose[..., None, :, :]
+        for js, par, M in self.levels:
+            f = (F.index_select(-3, par) @ M.to(q.dtype)) @ T.index_select(-3, js)
+            F = torch.cat([F, f], -3)
+        J = F.index_select(-3, self.frame_slot)
         G = J[..., self.geom_index, :, :] @ self.geom_mats.to(q.dtype)
         G = G.reshape(S, -1, 4, 4)
         return G[..., :3, 3], matrix_to_quaternion(G[..., :3, :3])

@@ -73,10 +73,10 @@ def quaternion_to_matrix(q: torch.Tensor) -> torch.Tensor:
 
 
 def _sqrt_positive_part(x: torch.Tensor) -> torch.Tensor:
-    ret = torch.zeros_like(x)
+    """sqrt(max(x, 0)) with a zero gradient where x <= 0 (elementwise selects, no
+    boolean indexing: that would synchronise the host with the device)."""
     pos = x > 0
-    ret[pos] = torch.sqrt(x[pos])
-    return ret
+    return torch.where(pos, torch.sqrt(torch.where(pos, x, torch.ones_like(x))), torch.zeros_like(x))
 
 
 def matrix_to_quaternion(m: torch.Tensor) -> torch.Tensor:
