@@ -71,23 +71,25 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
   const int G = build_groups(valid, key, min(kWG, ch.z - ch.y), sh.grp, gr, pos);
   if (valid) stage_record(rc, pos, q.st, q.m, q.mv, q.affine);
   const float dx = S.dx;
-  // contributions of record s to the nodes (i, j, 0..2): p2g_node_contrib's operations,
-  // with the k-independent ones done once (affine = {b.w, c.x, c.y | c.z, c.w, d.x | d.y,
-  // d.z, d.w}, rows summed left to right as mvec does)
+  // contributions of record s to the nodes (i, j, 0..2): w (m, mv + A d) with the
+  // affine matrix A = {b.w, c.x, c.y | c.z, c.w, d.x | d.y, d.z, d.w} and d = (i, j, k) dx
+  // - fx dx (p2g_node_contrib's terms), the k-independent parts once per column and the
+  // products accumulated with fused multiply-adds: w (mv + A_xy d_xy) + (w d_z) A_z
   auto contrib = [dx](const StencilRecs &r, int s, int i, int j, float (*acc)[4]) {
     const float4 a = r.f0[s], b = r.f1[s], c = r.f2[s], d = r.f3[s];
     const float wxy = bspline_wi(a.x, i) * bspline_wi(a.y, j);
     const float wz[3] = {bspline_w<0>(a.z), bspline_w<1>(a.z), bspline_w<2>(a.z)};
     const float d0 = ((float)i - a.x) * dx, d1 = ((float)j - a.y) * dx;
-    const float p0 = b.w * d0 + c.x * d1, p1 = c.z * d0 + c.w * d1, p2 = d.y * d0 + d.z * d1;
+    const float s0 = b.x + fmaf(c.x, d1, b.w * d0), s1 = b.y + fmaf(c.w, d1, c.z * d0),
+                s2 = b.z + fmaf(d.z, d1, d.y * d0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float w = wxy * wz[k];
-      const float d2 = ((float)k - a.z) * dx;
-      acc[k][0] += a.w * w;
-      acc[k][1] += (b.x + (p0 + c.y * d2)) * w;
-      acc[k][2] += (b.y + (p1 + d.x * d2)) * w;
-      acc[k][3] += (b.z + (p2 + d.w * d2)) * w;
+      const float wd = w * (((float)k - a.z) * dx);
+      acc[k][0] = fmaf(a.w, w, acc[k][0]);
+      acc[k][1] = fmaf(s0, w, fmaf(c.y, wd, acc[k][1]));
+      acc[k][2] = fmaf(s1, w, fmaf(d.x, wd, acc[k][2]));
+      acc[k][3] = fmaf(s2, w, fmaf(d.w, wd, acc[k][3]));
     }
   };
   float *gmmv = B.grid_mmv;
