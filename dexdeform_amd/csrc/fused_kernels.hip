@@ -33,9 +33,11 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
   __shared__ Groups gr;
   __shared__ int s_box[6];
   float *const tm = sh.tile[0], *const tx = sh.tile[1], *const ty = sh.tile[2], *const tz = sh.tile[3];
-  if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
   const int ci = chunk_index(B);
-  if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
+  if (ci < 0) {  // uniform: a fixed-size launch with fewer chunks
+    if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
+    return;
+  }
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
   const ddm_scene S = B.scenes[ch.x];
   if (MODE == kRecomputeUntaped && S.tape_mmv && S.tape_status[1] == 0) return;  // uniform: one scene per chunk
@@ -52,15 +54,14 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
     m33 U, V;
     f3 sig;
     svd3(Ft, U, sig, V);
-    m33 Fn;
-    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
-                 ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
-    if (temps) {
-      st9(S.tmp_Ft + 9 * (size_t)p, Ft);
+    if (temps) {  // stored before the stress so that their registers can retire early
       st9(S.tmp_U + 9 * (size_t)p, U);
       st9(S.tmp_V + 9 * (size_t)p, V);
       st3(S.tmp_sig + 3 * (size_t)p, sig);
     }
+    m33 Fn;
+    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+                 ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
     if (MODE == kForward || MODE == kForwardTaped) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     key = pack_base(q.st.base);
   } else {
@@ -136,6 +137,8 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
       flag_stencil_blocks(B, S, ch.x, b);
     }
   }
+  // at the end, where few registers are live (at the start it costs the kernel spills)
+  if (MODE == kForwardTaped && blockIdx.x == 0) tape_reset(B);
 }
 
 // ---------------------------------------------------------------- grid update
@@ -296,7 +299,6 @@ __global__ __launch_bounds__(kWG) void k_det_prep(ddm_batch B, ddm_det D, const 
       p2g_particle(x, ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                    ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
       if (temps) {
-        st9(S.tmp_Ft + 9 * (size_t)p, Ft);
         st9(S.tmp_U + 9 * (size_t)p, U);
         st9(S.tmp_V + 9 * (size_t)p, V);
         st3(S.tmp_sig + 3 * (size_t)p, sig);

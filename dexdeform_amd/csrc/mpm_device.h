@@ -533,11 +533,22 @@ DDM_HD m33 mul_atb(const m33 &a, const m33 &b) {  // a^T * b
   return r;
 }
 
-// F_tmp = (I + dt C) F  (integrator.cu:90)
+// F_tmp = (I + dt C) F  (integrator.cu:90). Written out here (the operations of mscale and
+// mmul, in their order) and never contracted, also in sources built with FMA contraction,
+// so every kernel that computes it gets the same bits.
 DDM_HD m33 advect_F(const m33 &F, const m33 &C, float dt) {
-  m33 A = mscale(C, dt);
-  A.a[0] += 1.f; A.a[4] += 1.f; A.a[8] += 1.f;
-  return mmul(A, F);
+#pragma clang fp contract(off)
+  float A[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) A[i] = C.a[i] * dt;
+  A[0] += 1.f; A[4] += 1.f; A[8] += 1.f;
+  m33 r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r.a[3 * i + j] = A[3 * i + 0] * F.a[0 + j] + A[3 * i + 1] * F.a[3 + j] + A[3 * i + 2] * F.a[6 + j];
+  return r;
 }
 
 // Per-particle P2G quantities (integrator.cu:340-375). Returns the affine matrix and

@@ -27,7 +27,7 @@ from typing import List, Optional
 import torch
 
 from .. import _lib
-from .engine import Engine, gc_paused
+from .engine import TEMP_FLOATS, Engine, gc_paused
 
 
 class _Particles:
@@ -65,7 +65,7 @@ class _Tapes:
         self.caps = [int(c) for c in caps]
         self.complete = engine.tape_complete(self.caps)  # no substep can overflow its snapshot
         nblk = sum(self.caps)
-        self.temps = torch.empty(slots, 30 * P, dtype=torch.float32, device=dev)
+        self.temps = torch.empty(slots, TEMP_FLOATS * P, dtype=torch.float32, device=dev)
         self.grid_mmv = torch.empty(slots, nblk * 256, dtype=torch.float32, device=dev)
         self.grid_v = torch.empty_like(self.grid_mmv)
         self.block_list = torch.empty(slots, nblk, dtype=torch.int32, device=dev)
@@ -83,7 +83,7 @@ class Rollout:
                  tape_blocks: Optional[int] = None):
         """tape=True keeps, for every substep of a segment, the SVD temporaries and the
         grid of the forward pass, so the adjoint substeps skip recomputing P2G and the
-        grid update (120 B per particle plus 2 KiB per active grid block per substep).
+        grid update (84 B per particle plus 2 KiB per active grid block per substep).
         tape_blocks bounds the blocks one scene's snapshot holds per substep (an int for
         every scene or a list per scene; default: Engine.tape_capacities); a scene whose substep lists more is recomputed by its
         adjoint instead (decided on the device)."""

@@ -51,7 +51,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .engine import CHUNK, SceneSpec, chunk_size_for, chunk_table, gc_paused
+from .engine import CHUNK, TEMP_FLOATS, SceneSpec, chunk_size_for, chunk_table, gc_paused
 from .workloads import needs_sort, spatial_order
 
 
@@ -317,7 +317,7 @@ def request_bytes(req: Request, K: Optional[int] = None) -> int:
     K = K or req.checkpoint_every or T
     n_states = (T // K + 1 if K != T else 0) + (K + 1) + 2
     per_state = req.n * 24 * 4
-    temps = req.n * 5 * 4 + K * (req.n * 30 * 4 + tape_capacity(req) * (2 * 1024 + 4) + 8)
+    temps = req.n * 5 * 4 + K * (req.n * TEMP_FLOATS * 4 + tape_capacity(req) * (2 * 1024 + 4) + 8)
     nb = max(req.spec.n_bodies, 1)
     poses = (T + 1) * nb * 7 * 4 * 2
     gd = req.spec.grid_dim
@@ -357,7 +357,7 @@ class _Active:
         # taped adjoint: per substep of a segment, the SVD temporaries and a grid snapshot
         # (written by the segment's taped forward pass, read by its adjoint substeps)
         self.tape_cap = tape_capacity(req)
-        self.tape_temps = torch.empty(self.K, 30 * n, dtype=torch.float32, device=dev)
+        self.tape_temps = torch.empty(self.K, TEMP_FLOATS * n, dtype=torch.float32, device=dev)
         self.tape_mmv = torch.empty(self.K, self.tape_cap * 256, dtype=torch.float32, device=dev)
         self.tape_v = torch.empty_like(self.tape_mmv)
         self.tape_list = torch.empty(self.K, self.tape_cap, dtype=torch.int32, device=dev)
@@ -410,7 +410,7 @@ class _Active:
 
     def _temps_of(self, r, j):
         base, n, f4 = int(self.tape_temps[j].data_ptr()), self.req.n, 4
-        r["tmp_Ft"], r["tmp_U"], r["tmp_V"], r["tmp_sig"] = base, base + 9 * n * f4, base + 18 * n * f4, base + 27 * n * f4
+        r["tmp_U"], r["tmp_V"], r["tmp_sig"] = base, base + 9 * n * f4, base + 18 * n * f4
 
     # ---- operation schedule: ("pre", callable) run before the tick's launches,
     #      ("F", j, t, taped) forward substep seg[j] -> seg[j+1] at substep t (taped: it
@@ -481,8 +481,8 @@ class _Active:
         recs = np.repeat(np.array([self.record], dtype=_lib.SCENE_DTYPE), K)
         slot = lambda t: np.uint64(t.data_ptr()) + j * np.uint64(t.stride(0) * t.element_size())
         temps = slot(self.tape_temps)
-        recs["tmp_Ft"], recs["tmp_U"] = temps, temps + np.uint64(9 * n * f4)
-        recs["tmp_V"], recs["tmp_sig"] = temps + np.uint64(18 * n * f4), temps + np.uint64(27 * n * f4)
+        recs["tmp_U"], recs["tmp_V"] = temps, temps + np.uint64(9 * n * f4)
+        recs["tmp_sig"] = temps + np.uint64(18 * n * f4)
         recs["tape_mmv"], recs["tape_v"] = slot(self.tape_mmv), slot(self.tape_v)
         recs["tape_list"], recs["tape_status"] = slot(self.tape_list), slot(self.tape_status)
         recs["tape_capacity"] = self.tape_cap

@@ -15,7 +15,7 @@ from __future__ import annotations
 import torch
 from torch.autograd import Function
 
-from .engine import gc_paused
+from .engine import TEMP_FLOATS, gc_paused
 
 
 class GradModel:
@@ -27,7 +27,7 @@ class GradModel:
 
         tape (with graphs): the forward substeps also keep the SVD temporaries and the
         grid snapshot of every substep, and the adjoint substeps restore them instead of
-        recomputing P2G and the grid update (120 B per particle plus ~2 KiB per active node
+        recomputing P2G and the grid update (84 B per particle plus ~2 KiB per active node
         block, per substep of the simulator's max_steps). None: on when that memory is at
         most 16 GiB and a quarter of the device. The adjoint of an environment step reads
         what its own forward wrote, as it reads the states that forward wrote."""
@@ -151,7 +151,7 @@ class GradModel:
             eng, sim = self.sim.engine, self.sim
             P = eng.n_particles_total
             slots = sim.max_steps - 1
-            need = lambda caps: slots * (30 * P * 4 + sum(caps) * (2 * 256 * 4 + 4) + 8)  # noqa: E731
+            need = lambda caps: slots * (TEMP_FLOATS * P * 4 + sum(caps) * (2 * 256 * 4 + 4) + 8)  # noqa: E731
             budget = min(16 << 30, torch.cuda.get_device_properties(sim.device).total_memory // 4)
             caps = eng.tape_capacities()
             if need(caps) > budget:  # whole-grid snapshots for every substep do not fit: bounded ones

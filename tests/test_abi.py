@@ -38,12 +38,12 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     import ctypes
     from dexdeform_amd import _lib
-    assert ctypes.sizeof(_lib.DdmScene) == 176
+    assert ctypes.sizeof(_lib.DdmScene) == 168
     assert ctypes.sizeof(_lib.DdmState) == 6 * 8
     assert ctypes.sizeof(_lib.DdmBatch) == 16 + 11 * 8 + 8
     assert ctypes.sizeof(_lib.DdmDet) == 10 * 8
     hdr = open(HEADER).read()
-    assert "} ddm_scene;                   /* 176 bytes */" in hdr
+    assert "} ddm_scene;                   /* 168 bytes */" in hdr
 
 
 # Every symbol the reference's ctypes layer binds (mpm/types.py:103-290) or calls

@@ -257,7 +257,7 @@ def test_tick_graph_buckets():
 
 def test_tick_table_layout_is_compact_aligned_and_disjoint():
     """A tick key's tables: every part 64-byte aligned, parts disjoint and in order,
-    capacities as the launch sizes need (records 176 B and state records 48 B per scene,
+    capacities as the launch sizes need (records 168 B and state records 48 B per scene,
     16 B per chunk), and the span a few KB for one small request."""
     from dexdeform_amd import _lib
     from dexdeform_amd.mpm.serve import _TickTables
