@@ -36,11 +36,11 @@ KERNEL_BYTES = {
     "p2g_fwd": 152,      # read x,v,F,C + mass,vol,mu/lam/yield (116) ; write F' (36)
     "grid_fwd": 0,
     "g2p_fwd": 72,       # read x (12) ; write x',v',C' (60)
-    "p2g_recompute": 200,  # read x,v,F,C + material (116) ; write U,V,sigma (84)
+    "p2g_recompute": 160,  # read x,v,F,C + material (116) ; write U,V (quaternions), sigma (44)
     "grid_recompute": 0,
     "g2p_adj": 108,      # read x,v',dx',dv',dC' (84) ; read+write dx (24)
     "grid_adj": 0,
-    "p2g_adj": 428,      # read x,v,F,C,material,U,V,sigma,dF' (236) ; read+write dx,dv,dF,dC (192)
+    "p2g_adj": 388,      # read x,v,F,C,material,U,V,sigma,dF' (196) ; read+write dx,dv,dF,dC (192)
 }
 # HBM bytes per particle per launch measured on the default workload (64 scenes, seed 7)
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
@@ -50,12 +50,12 @@ TRAFFIC_BYTES = {
     "p2g_fwd": 272, "grid_fwd": 10, "g2p_fwd": 75, "p2g_recompute": 311, "grid_recompute": 7, "g2p_adj": 148,
     "grid_adj": 18, "p2g_adj": 442,
     # taped rollout
-    "taped_p2g_fwd": 335, "taped_grid_fwd": 17, "taped_g2p_fwd": 75, "taped_restore": 13, "taped_g2p_adj": 148,
-    "taped_grid_adj": 18, "taped_p2g_adj": 399,
+    "taped_p2g_fwd": 263, "taped_grid_fwd": 17, "taped_g2p_fwd": 75, "taped_restore": 13, "taped_g2p_adj": 148,
+    "taped_grid_adj": 18, "taped_p2g_adj": 358,
 }
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
-# the taped rollout: its forward P2G also writes the temporaries (84 B per particle) and
+# the taped rollout: its forward P2G also writes the temporaries (44 B per particle) and
 # its adjoint restores the grid snapshot (no recompute unless the snapshot overflowed).
 # The tape is an implementation choice, not part of the path's algorithmic traffic: like
 # checkpoint recompute it is overhead, so the taped P2G is credited the same 152 B.

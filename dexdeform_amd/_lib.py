@@ -37,7 +37,7 @@ class DdmScene(ctypes.Structure):
         ("dx", c_float), ("inv_dx", c_float), ("dt", c_float), ("ground_friction", c_float),
         ("ground_height", c_float), ("gravity", c_float * 3),
         ("mass", c_void_p), ("vol", c_void_p), ("mu_lam_yield", c_void_p), ("body_prim", c_void_p),
-        ("body_args", c_void_p), ("tmp_U", c_void_p), ("tmp_V", c_void_p), ("tmp_sig", c_void_p),
+        ("body_args", c_void_p), ("tmp_qu", c_void_p), ("tmp_qv", c_void_p), ("tmp_sig", c_void_p),
         ("tape_mmv", c_void_p), ("tape_v", c_void_p), ("tape_list", c_void_p), ("tape_status", c_void_p),
         ("tape_capacity", c_int32), ("reserved", c_int32),
     ]
@@ -49,7 +49,7 @@ SCENE_DTYPE = np.dtype([
     ("dx", "<f4"), ("inv_dx", "<f4"), ("dt", "<f4"), ("ground_friction", "<f4"),
     ("ground_height", "<f4"), ("gravity", "<f4", (3,)),
     ("mass", "<u8"), ("vol", "<u8"), ("mu_lam_yield", "<u8"), ("body_prim", "<u8"), ("body_args", "<u8"),
-    ("tmp_U", "<u8"), ("tmp_V", "<u8"), ("tmp_sig", "<u8"),
+    ("tmp_qu", "<u8"), ("tmp_qv", "<u8"), ("tmp_sig", "<u8"),
     ("tape_mmv", "<u8"), ("tape_v", "<u8"), ("tape_list", "<u8"), ("tape_status", "<u8"),
     ("tape_capacity", "<i4"), ("reserved", "<i4"),
 ])

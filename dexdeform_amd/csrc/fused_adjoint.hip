@@ -364,8 +364,9 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
   BlockedGatherDMMV gat{reinterpret_cast<const float4 *>(B.grid_dmmv), &S};
   m33 F = ld9(cur.F + p9), C = ld9(cur.C + p9);
   // F_tmp as the forward P2G computed it (advect_F is evaluated without contraction here
-  // too, so it is the same bits); the SVD of it comes from the temporaries
-  const m33 Ft = advect_F(F, C, S.dt), U = ld9(S.tmp_U + p9), V = ld9(S.tmp_V + p9);
+  // too, so it is the same bits); its SVD from the temporaries (U and V as quaternions)
+  const m33 Ft = advect_F(F, C, S.dt), U = quat_mat(ldq(S.tmp_qu + 4 * (size_t)p)),
+            V = quat_mat(ldq(S.tmp_qv + 4 * (size_t)p));
   f3 sig = ld3(S.tmp_sig + p3);
   P2GAdj a;
   p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],

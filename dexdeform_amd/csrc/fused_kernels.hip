@@ -53,10 +53,11 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
     m33 Ft = advect_F(F, C, S.dt);
     m33 U, V;
     f3 sig;
-    svd3(Ft, U, sig, V);
+    q4 qu, qv;
+    svd3(Ft, U, sig, V, qu, qv);
     if (temps) {  // stored before the stress so that their registers can retire early
-      st9(S.tmp_U + 9 * (size_t)p, U);
-      st9(S.tmp_V + 9 * (size_t)p, V);
+      stq(S.tmp_qu + 4 * (size_t)p, qu);
+      stq(S.tmp_qv + 4 * (size_t)p, qv);
       st3(S.tmp_sig + 3 * (size_t)p, sig);
     }
     m33 Fn;
@@ -293,14 +294,15 @@ __global__ __launch_bounds__(kWG) void k_det_prep(ddm_batch B, ddm_det D, const 
       m33 Ft = advect_F(F, C, S.dt);
       m33 U, V;
       f3 sig;
-      svd3(Ft, U, sig, V);
+      q4 qu, qv;
+      svd3(Ft, U, sig, V, qu, qv);
       m33 Fn;
       P2GParticle q;
       p2g_particle(x, ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                    ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
       if (temps) {
-        st9(S.tmp_U + 9 * (size_t)p, U);
-        st9(S.tmp_V + 9 * (size_t)p, V);
+        stq(S.tmp_qu + 4 * (size_t)p, qu);
+        stq(S.tmp_qv + 4 * (size_t)p, qv);
         st3(S.tmp_sig + 3 * (size_t)p, sig);
       }
       if (MODE == kForward || MODE == kForwardTaped) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);

@@ -125,6 +125,7 @@ DDM_HD void add9(float *p, const m33 &m) {
   for (int i = 0; i < 9; ++i) p[i] += m.a[i];
 }
 DDM_HD q4 ldq(const float *p) { return q4{p[0], p[1], p[2], p[3]}; }
+DDM_HD void stq(float *p, q4 q) { p[0] = q.w; p[1] = q.x; p[2] = q.y; p[3] = q.z; }
 
 // ------------------------------------------------------------------ rotations (quat.h)
 DDM_HD q4 qconj(q4 q) { return q4{q.w, -q.x, -q.y, -q.z}; }
@@ -338,8 +339,31 @@ DDM_HD void qr_rot(double a1, double a2, double &ch, double &sh) {  // svd.h:248
 }
 }  // namespace svd
 
-// F = U diag(sig) V^T ; F given in FP32, factors returned in FP32 (svd.h:390-413)
-DDM_HD void svd3(const m33 &F, m33 &U, f3 &sig, m33 &V) {
+// q (x) r for r = (1 + s e_K) / sqrt(2), the rotation by s 90 degrees about axis K:
+// right-multiplying the rotation matrix of q by that of r
+template <int K>
+DDM_HD q4 qmul_quarter(q4 q, float s) {
+  constexpr float h = 0.70710678118654752f;
+  if (K == 2) return q4{(q.w - s * q.z) * h, (q.x + s * q.y) * h, (q.y - s * q.x) * h, (q.z + s * q.w) * h};
+  if (K == 1) return q4{(q.w - s * q.y) * h, (q.x - s * q.z) * h, (q.y + s * q.w) * h, (q.z + s * q.x) * h};
+  return q4{(q.w - s * q.x) * h, (q.x + s * q.w) * h, (q.y + s * q.z) * h, (q.z - s * q.y) * h};
+}
+DDM_HD q4 qsel(bool c, q4 a, q4 b) { return c ? a : b; }
+// Rotation matrix of a unit quaternion (w, x, y, z), in the form svd3 builds V with
+DDM_HD m33 quat_mat(q4 q) {
+  const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z, xz = q.x * q.z, xy = q.x * q.y, yz = q.y * q.z,
+              wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+  return m33{{1.f - 2.f * (yy + zz), 2.f * (xy - wz), 2.f * (xz + wy), 2.f * (xy + wz), 1.f - 2.f * (xx + zz),
+              2.f * (yz - wx), 2.f * (xz - wy), 2.f * (yz + wx), 1.f - 2.f * (xx + yy)}};
+}
+
+// F = U diag(sig) V^T ; F given in FP32, factors returned in FP32 (svd.h:390-413).
+// Also returns U and V as quaternions (qu, qv; quat_mat(qu) = U and quat_mat(qv) = V to
+// FP32 rounding), the compact form the adjoint's tape keeps: V is the Jacobi quaternion
+// times a quarter turn per column swap of the sort (a swap with one sign flip is a
+// rotation by +-90 degrees about the third axis), U = Rz Ry Rx of the three Givens
+// rotations, whose half-angle cosines and sines the QR already has.
+DDM_HD void svd3(const m33 &F, m33 &U, f3 &sig, m33 &V, q4 &qu, q4 &qv) {
 #pragma clang fp contract(fast)
   using namespace svd;
   double a[9];
@@ -376,17 +400,22 @@ DDM_HD void svd3(const m33 &F, m33 &U, f3 &sig, m33 &V) {
   double r1 = b[0] * b[0] + b[3] * b[3] + b[6] * b[6];
   double r2 = b[1] * b[1] + b[4] * b[4] + b[7] * b[7];
   double r3 = b[2] * b[2] + b[5] * b[5] + b[8] * b[8];
+  q4 qs{(float)q[3], (float)q[0], (float)q[1], (float)q[2]};
   bool c = r1 < r2;
 #pragma unroll
   for (int r = 0; r < 3; ++r) { cnswap(c, b[3 * r], b[3 * r + 1]); cnswap(c, v[3 * r], v[3 * r + 1]); }
   cswap(c, r1, r2);
+  qs = qsel(c, qmul_quarter<2>(qs, 1.f), qs);  // columns (0, 1): +90 degrees about z
   c = r1 < r3;
 #pragma unroll
   for (int r = 0; r < 3; ++r) { cnswap(c, b[3 * r], b[3 * r + 2]); cnswap(c, v[3 * r], v[3 * r + 2]); }
   cswap(c, r1, r3);
+  qs = qsel(c, qmul_quarter<1>(qs, -1.f), qs);  // columns (0, 2): -90 degrees about y
   c = r2 < r3;
 #pragma unroll
   for (int r = 0; r < 3; ++r) { cnswap(c, b[3 * r + 1], b[3 * r + 2]); cnswap(c, v[3 * r + 1], v[3 * r + 2]); }
+  qs = qsel(c, qmul_quarter<0>(qs, 1.f), qs);  // columns (1, 2): +90 degrees about x
+  qv = qs;
   // Givens QR of B: three rotations annihilating (2,1), (3,1), (3,2)
   double ch1, sh1, ch2, sh2, ch3, sh3, ca, sb;
   qr_rot(b[0], b[3], ch1, sh1);
@@ -421,6 +450,16 @@ DDM_HD void svd3(const m33 &F, m33 &U, f3 &sig, m33 &V) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) { U.a[i] = (float)u[i]; V.a[i] = (float)v[i]; }
   sig = f3{(float)c11, (float)d22, (float)d33};
+  {  // U = G1^T G2^T G3^T: rotations by the Givens angles about z, -y and x
+    const float c1 = (float)ch1, s1 = (float)sh1, c2 = (float)ch2, s2 = (float)sh2, c3 = (float)ch3,
+                s3 = (float)sh3;
+    const float pw = c1 * c2, px = s1 * s2, py = -c1 * s2, pz = s1 * c2;  // (c1, s1 z) (x) (c2, -s2 y)
+    qu = q4{pw * c3 - px * s3, pw * s3 + px * c3, py * c3 + pz * s3, pz * c3 - py * s3};  // (x) (c3, s3 x)
+  }
+}
+DDM_HD void svd3(const m33 &F, m33 &U, f3 &sig, m33 &V) {
+  q4 qu, qv;
+  svd3(F, U, sig, V, qu, qv);
 }
 
 // ------------------------------------------------------------------ particle physics

@@ -25,9 +25,9 @@ import torch
 from .. import _lib
 
 CHUNK = 256  # particles per workgroup in the particle kernels
-# floats per particle of one substep's SVD temporaries (U, V: 9 each, sigma: 3); the adjoint
-# recomputes F_tmp = (I + dt C) F from the state
-TEMP_FLOATS = 21
+# floats per particle of one substep's SVD temporaries (U and V as quaternions: 4 each,
+# sigma: 3); the adjoint recomputes F_tmp = (I + dt C) F from the state
+TEMP_FLOATS = 11
 
 
 @dataclass
@@ -207,11 +207,11 @@ class Engine:
         if with_grad:
             self.grid_dv = z(G, 4)
             self.grid_dmmv = z(G, 4)
-            self.tmp_U = z(P, 9)
-            self.tmp_V = z(P, 9)
+            self.tmp_qu = z(P, 4)
+            self.tmp_qv = z(P, 4)
             self.tmp_sig = z(P, 3)
         else:
-            self.grid_dv = self.grid_dmmv = self.tmp_U = self.tmp_V = self.tmp_sig = None
+            self.grid_dv = self.grid_dmmv = self.tmp_qu = self.tmp_qv = self.tmp_sig = None
 
         self.deterministic = bool(deterministic)
         self._det = None
@@ -250,8 +250,8 @@ class Engine:
                 tab[i]["body_prim"] = self.body_prim.data_ptr() + 4 * bb * f4
                 tab[i]["body_args"] = self.body_args.data_ptr() + 4 * bb * f4
             if with_grad:
-                tab[i]["tmp_U"] = self.tmp_U.data_ptr() + 9 * pb * f4
-                tab[i]["tmp_V"] = self.tmp_V.data_ptr() + 9 * pb * f4
+                tab[i]["tmp_qu"] = self.tmp_qu.data_ptr() + 4 * pb * f4
+                tab[i]["tmp_qv"] = self.tmp_qv.data_ptr() + 4 * pb * f4
                 tab[i]["tmp_sig"] = self.tmp_sig.data_ptr() + 3 * pb * f4
         self._scene_np = tab
         self.scene_table = torch.as_tensor(tab.view(np.uint8).copy(), device=dev)
@@ -385,8 +385,8 @@ class Engine:
         return len(caps) == len(self.scenes) and all(int(c) >= b for c, b in zip(caps, self.grid_blocks()))
 
     def taped_table(self, temps, tape_mmv, tape_v, tape_list, tape_status, caps) -> torch.Tensor:
-        """Scene table of one taped substep: each scene's U/V/sigma temporaries in
-        ``temps`` (TEMP_FLOATS * n_particles_total floats: U | V | sigma) and its grid
+        """Scene table of one taped substep: each scene's SVD temporaries in ``temps``
+        (TEMP_FLOATS * n_particles_total floats: U and V as quaternions | sigma) and its grid
         snapshot in the next caps[i] blocks of tape_mmv / tape_v / tape_list (scenes in
         order), with two status ints per scene in ``tape_status``."""
         P = self.n_particles_total
@@ -400,9 +400,9 @@ class Engine:
         off = 0
         for i in range(len(tab)):
             pb = int(self.p_begin[i])
-            tab[i]["tmp_U"] = base + 9 * pb * f4
-            tab[i]["tmp_V"] = base + (9 * P + 9 * pb) * f4
-            tab[i]["tmp_sig"] = base + (18 * P + 3 * pb) * f4
+            tab[i]["tmp_qu"] = base + 4 * pb * f4
+            tab[i]["tmp_qv"] = base + (4 * P + 4 * pb) * f4
+            tab[i]["tmp_sig"] = base + (8 * P + 3 * pb) * f4
             tab[i]["tape_mmv"] = int(tape_mmv.data_ptr()) + off * 256 * f4
             tab[i]["tape_v"] = int(tape_v.data_ptr()) + off * 256 * f4
             tab[i]["tape_list"] = int(tape_list.data_ptr()) + off * 4
@@ -446,7 +446,7 @@ class Engine:
 
     def nbytes(self) -> int:
         ts = [self.mass, self.vol, self.mu_lam_yield, self.body_prim, self.body_args, self.grid_mmv, self.grid_v,
-              self.block_flags, self.block_list, self.block_count, self.grid_dv, self.grid_dmmv, self.tmp_U, self.tmp_V,
+              self.block_flags, self.block_list, self.block_count, self.grid_dv, self.grid_dmmv, self.tmp_qu, self.tmp_qv,
               self.tmp_sig]
         if self._det is not None:
             ts += list(self.det_buffers.values())

@@ -83,7 +83,7 @@ class Rollout:
                  tape_blocks: Optional[int] = None):
         """tape=True keeps, for every substep of a segment, the SVD temporaries and the
         grid of the forward pass, so the adjoint substeps skip recomputing P2G and the
-        grid update (84 B per particle plus 2 KiB per active grid block per substep).
+        grid update (44 B per particle plus 2 KiB per active grid block per substep).
         tape_blocks bounds the blocks one scene's snapshot holds per substep (an int for
         every scene or a list per scene; default: Engine.tape_capacities); a scene whose substep lists more is recomputed by its
         adjoint instead (decided on the device)."""

@@ -410,7 +410,7 @@ class _Active:
 
     def _temps_of(self, r, j):
         base, n, f4 = int(self.tape_temps[j].data_ptr()), self.req.n, 4
-        r["tmp_U"], r["tmp_V"], r["tmp_sig"] = base, base + 9 * n * f4, base + 18 * n * f4
+        r["tmp_qu"], r["tmp_qv"], r["tmp_sig"] = base, base + 4 * n * f4, base + 8 * n * f4
 
     # ---- operation schedule: ("pre", callable) run before the tick's launches,
     #      ("F", j, t, taped) forward substep seg[j] -> seg[j+1] at substep t (taped: it
@@ -481,8 +481,8 @@ class _Active:
         recs = np.repeat(np.array([self.record], dtype=_lib.SCENE_DTYPE), K)
         slot = lambda t: np.uint64(t.data_ptr()) + j * np.uint64(t.stride(0) * t.element_size())
         temps = slot(self.tape_temps)
-        recs["tmp_U"], recs["tmp_V"] = temps, temps + np.uint64(9 * n * f4)
-        recs["tmp_sig"] = temps + np.uint64(18 * n * f4)
+        recs["tmp_qu"], recs["tmp_qv"] = temps, temps + np.uint64(4 * n * f4)
+        recs["tmp_sig"] = temps + np.uint64(8 * n * f4)
         recs["tape_mmv"], recs["tape_v"] = slot(self.tape_mmv), slot(self.tape_v)
         recs["tape_list"], recs["tape_status"] = slot(self.tape_list), slot(self.tape_status)
         recs["tape_capacity"] = self.tape_cap

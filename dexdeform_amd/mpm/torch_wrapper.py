@@ -27,7 +27,7 @@ class GradModel:
 
         tape (with graphs): the forward substeps also keep the SVD temporaries and the
         grid snapshot of every substep, and the adjoint substeps restore them instead of
-        recomputing P2G and the grid update (84 B per particle plus ~2 KiB per active node
+        recomputing P2G and the grid update (44 B per particle plus ~2 KiB per active node
         block, per substep of the simulator's max_steps). None: on when that memory is at
         most 16 GiB and a quarter of the device. The adjoint of an environment step reads
         what its own forward wrote, as it reads the states that forward wrote."""

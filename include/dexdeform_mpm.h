@@ -179,8 +179,9 @@ typedef struct ddm_scene {
   const float *mu_lam_yield;   /* [p_count][3] */
   const float *body_prim;      /* [n_bodies][4] (type, friction, softness, round) */
   const float *body_args;      /* [n_bodies][4] */
-  float *tmp_U, *tmp_V, *tmp_sig;  /* SVD of F_tmp = (I + dt C) F: [p_count][9],[..][9],[..][3];
-                                      backward only (F_tmp itself is recomputed from F and C) */
+  float *tmp_qu, *tmp_qv, *tmp_sig; /* SVD of F_tmp = (I + dt C) F for the backward: U and V as
+                                      unit quaternions (w, x, y, z) [p_count][4] each, sigma
+                                      [p_count][3] (F_tmp is recomputed from F and C) */
   /* Grid snapshot of this scene for one substep, used only by the taped entry points
    * (no counterpart in the reference, whose backward always recomputes P2G and the grid
    * update, simulator.py:573-585). NULL tape_mmv = no snapshot. */
@@ -277,7 +278,7 @@ int ddm_substep_backward_timed(const ddm_batch *b, const ddm_state *cur, const d
 
 /* Taped substeps: the forward keeps what its adjoint needs, per scene, in the scene
  * records (give each taped substep its own scene table): the SVD temporaries in
- * tmp_U/tmp_V/tmp_sig and a snapshot of the scene's listed grid blocks in tape_*.
+ * tmp_qu/tmp_qv/tmp_sig and a snapshot of the scene's listed grid blocks in tape_*.
  * No counterpart in the reference; for callers with the memory to keep them (120 B per
  * particle plus 2 KiB per active block per substep). Scenes without a snapshot (NULL
  * tape_mmv) run as in ddm_substep_forward. */

@@ -306,6 +306,62 @@ def test_stage_launchers_forward_and_backward(plain):
         assert e <= max(1e-3, 2 * noise), (name, e)
 
 
+def _quat_mat(q):
+    """Rotation matrices of quaternions (w, x, y, z), float64."""
+    w, x, y, z = (q[:, i].astype(np.float64) for i in range(4))
+    return np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                     2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                     2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], axis=1)
+
+
+def test_tape_quaternions_reproduce_the_svd_factors(plain):
+    """The adjoint's temporaries keep U and V of F_tmp's SVD as quaternions. Rebuilt as
+    matrices they equal the factors that the reference-layout compute_svd launcher writes
+    for the same F and C (same F_tmp bits, same SVD code) to FP32 rounding, as does sigma,
+    on every particle: a developed state plus random F with reflections,
+    near-equal and unordered singular values (every column swap of the sort). A
+    transposed, conjugated or sign-flipped encoding of either factor is off by O(1)."""
+    sc, st = plain["sc"], plain["states"][20]
+    rng = np.random.default_rng(11)
+    F = st.F.copy().reshape(sc.n, 3, 3)
+    C = st.C.copy().reshape(sc.n, 3, 3)
+    k = 2000
+    Q1 = np.linalg.qr(rng.normal(size=(k, 3, 3)))[0]
+    Q2 = np.linalg.qr(rng.normal(size=(k, 3, 3)))[0]
+    sv = rng.uniform(0.5, 1.5, size=(k, 3))
+    sv[: k // 4, 1] = sv[: k // 4, 0] * (1 + 1e-6)  # near-equal pairs
+    F[:k] = Q1 @ (sv[:, :, None] * Q2)  # random order of sizes, det of either sign
+    C[:k] = rng.normal(scale=2.0, size=(k, 3, 3))
+    F = F.astype(np.float32).reshape(sc.n, 9)
+    C = C.astype(np.float32).reshape(sc.n, 9)
+    dev = _dev()
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)  # noqa: E731
+    Fd, Cd = t(F), t(C)
+    Ft, U, V, sig = (torch.zeros(sc.n, c, dtype=torch.float32, device=dev) for c in (9, 9, 9, 3))
+    from dexdeform_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.compute_svd(Fd.data_ptr(), Cd.data_ptr(), Ft.data_ptr(), U.data_ptr(), V.data_ptr(),
+                               sig.data_ptr(), float(sc.spec.dt), sc.n, _lib.stream_ptr()), "compute_svd")
+
+    eng = _engine(sc)
+    cur, nxt, gcur, gnxt = eng.new_state(), eng.new_state(), eng.new_grad(), eng.new_grad()
+    _to_buf(st, cur)
+    _to_buf(st, nxt)
+    cur.F.copy_(Fd.reshape(cur.F.shape))
+    cur.C.copy_(Cd.reshape(cur.C.shape))
+    eng.backward(cur, nxt, gcur, gnxt)  # its recompute P2G writes the temporaries
+    torch.cuda.synchronize()
+    Un, Vn, sn = (a.cpu().numpy() for a in (U, V, sig))
+    eu = np.abs(_quat_mat(eng.tmp_qu.cpu().numpy()) - Un).max()
+    ev = np.abs(_quat_mat(eng.tmp_qv.cpu().numpy()) - Vn).max()
+    print(f"tape quaternions: max |U - quat_mat(qu)| {eu:.2e}, max |V - quat_mat(qv)| {ev:.2e}")
+    assert eu <= 2e-6 and ev <= 2e-6, (eu, ev)
+    # sigma: the same FP64 values rounded to FP32 (at most one FP32 ulp apart, should the
+    # compiler fuse an FP64 multiply-add differently in the two kernels)
+    es = np.abs(eng.tmp_sig.cpu().numpy() - sn).max() / np.abs(sn).max()
+    assert es <= 2.4e-7, es
+
+
 def test_observation_kernels_and_adjoints(plain):
     """compute_dist (particle-to-primitive SDF) and particle2mass (grid mass) launchers,
     forward and adjoint, against the oracle (relative L2 error <= 1e-5)."""
