@@ -61,12 +61,13 @@ BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"
 # checkpoint recompute it is overhead, so the taped P2G is credited the same 152 B.
 KERNEL_BYTES_TAPED = dict(KERNEL_BYTES, restore=0)
 # VALU issue: each of a CU's 4 SIMDs issues one wave64 VALU instruction per 4 cycles, at
-# the 2.4 GHz peak engine clock on 256 CUs (FP64 instructions take 2 issue slots)
+# the 2.4 GHz peak engine clock on 256 CUs (vector FP64 issues at the same rate: 78.6
+# TFLOP/s is 16 FMA lanes per SIMD per cycle)
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 # VALU wave-instructions per wave (64 particles) and the VALUBusy fraction per kernel,
 # from rocprofv3 SQ_INSTS_VALU / SQ_WAVES and SQ_ACTIVE_INST_VALU on the default workload
-VALU_INSTR_PER_WAVE = {"p2g_fwd": 5331, "g2p_fwd": 1265, "g2p_adj": 3607, "p2g_adj": 3476}
-VALU_BUSY = {"p2g_fwd": 0.70, "g2p_fwd": 0.89, "g2p_adj": 0.62, "p2g_adj": 0.72}
+VALU_INSTR_PER_WAVE = {"p2g_fwd": 4039, "g2p_fwd": 1265, "g2p_adj": 3607, "p2g_adj": 3532}
+VALU_BUSY = {"p2g_fwd": 0.58, "g2p_fwd": 0.93, "g2p_adj": 0.62, "p2g_adj": 0.76}
 FWD_NAMES_TAPED = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES_TAPED = ["restore", "g2p_adj", "grid_adj", "p2g_adj"]
 

@@ -362,17 +362,19 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
   const int dim[3] = {S.dim[0], S.dim[1], S.dim[2]};
   const size_t p3 = 3 * (size_t)p, p9 = 9 * (size_t)p;
   BlockedGatherDMMV gat{reinterpret_cast<const float4 *>(B.grid_dmmv), &S};
-  m33 F = ld9(cur.F + p9), C = ld9(cur.C + p9);
+  const float *Fp = cur.F + p9;
+  const m33 C = ld9(cur.C + p9);
   // F_tmp as the forward P2G computed it (advect_F is evaluated without contraction here
   // too, so it is the same bits); its SVD from the temporaries (U and V as quaternions)
-  const m33 Ft = advect_F(F, C, S.dt), U = quat_mat(ldq(S.tmp_qu + 4 * (size_t)p)),
+  const m33 Ft = advect_F(ld9(Fp), C, S.dt), U = quat_mat(ldq(S.tmp_qu + 4 * (size_t)p)),
             V = quat_mat(ldq(S.tmp_qv + 4 * (size_t)p));
   f3 sig = ld3(S.tmp_sig + p3);
   P2GAdj a;
   p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                    ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
   m33 gFt, gF, gC;
-  svd_adj(F, C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
+  asm volatile("" : "+v"(Fp));  // F is read again (from the cache) rather than held across the P2G adjoint
+  svd_adj(ld9(Fp), C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
   const bool overwrite = B.options & DDM_GRAD_OVERWRITE;
   add3(cur_grad.x + p3, a.x);  // onto the G2P adjoint's contribution of this substep
   if (overwrite) st3(cur_grad.v + p3, a.v);

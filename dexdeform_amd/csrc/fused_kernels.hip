@@ -49,8 +49,9 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
   P2GParticle q;
   int key = 0;
   if (valid) {
-    m33 F = ld9(cur.F + 9 * (size_t)p), C = ld9(cur.C + 9 * (size_t)p);
-    m33 Ft = advect_F(F, C, S.dt);
+    const float *Cp = cur.C + 9 * (size_t)p;
+    m33 F = ld9(cur.F + 9 * (size_t)p);
+    m33 Ft = advect_F(F, ld9(Cp), S.dt);
     m33 U, V;
     f3 sig;
     q4 qu, qv;
@@ -60,8 +61,9 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
       stq(S.tmp_qv + 4 * (size_t)p, qv);
       st3(S.tmp_sig + 3 * (size_t)p, sig);
     }
+    asm volatile("" : "+v"(Cp));  // C is read again (from the cache) rather than held across the SVD
     m33 Fn;
-    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), ld9(Cp), Ft, U, sig, V, S.mass[p], S.vol[p],
                  ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
     if (MODE == kForward || MODE == kForwardTaped) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     key = pack_base(q.st.base);
