@@ -15,7 +15,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
                                                   const ddm_state *__restrict__ next_tab,
                                                   const ddm_state *__restrict__ cur_grad_tab,
                                                   const ddm_state *__restrict__ next_grad_tab) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
   float4 *dmmv = reinterpret_cast<float4 *>(B.grid_dmmv);
@@ -36,9 +36,11 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
     const unsigned long long near = near_bodies(P, c.ix, c.iy, c.iz, lane);  // others: no contact here
     const f3 gx = node_position(c.ix, c.iy, c.iz, P.dx);
     const int mode = S.adjoint_mode;
-    // forward replay keeping the velocities entering each contact (integrator.cu:678-732)
+    // forward replay keeping the velocities entering the node's first kTapeMax contacts
+    // (integrator.cu:678-732) and the set of bodies in contact (bit b of hit: contact()
+    // returned true, i.e. in_contact held for that body at this node)
     f3 tape_v[kTapeMax];
-    int tape_b[kTapeMax];
+    unsigned long long hit = 0ull;
     int nt = 0;
     f3 v0{0.f, 0.f, 0.f}, g{0.f, 0.f, 0.f};
     if (act) {
@@ -48,76 +50,70 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
       for (int b = 0; b < P.nb; ++b) {
         if (!((near >> b) & 1ull)) continue;
         f3 vin = v;
-        if (contact(v, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b), P.dt) && nt < kTapeMax) {
-          tape_v[nt] = vin;
-          tape_b[nt] = b;
-          ++nt;
+        if (contact(v, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b), P.dt)) {
+          hit |= 1ull << b;
+          if (nt < kTapeMax) tape_v[nt] = vin;
+          ++nt;  // counts every contact; the tape keeps the first kTapeMax
         }
       }
       g = boundary_adj(v, f3{d.x, d.y, d.z}, c.ix, c.iy, c.iz, P, mode);
     }
-    // bodies in reverse order, uniformly across the wave so pose gradients can be reduced
+    // bodies in reverse order, uniformly across the wave so pose gradients can be reduced;
+    // a body no node of the block touched is skipped by the whole wave
     for (int b = P.nb - 1; b >= 0; --b) {
-      PoseGrad pg = posegrad_zero();
-      bool mine = false;
-      if (act && ((near >> b) & 1ull)) {
-        Body Bd = load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b);
-        float infl;
-        if (in_contact(prim_sdf(Bd.prim, Bd.args, to_body(Bd.pos, Bd.rot, gx)), Bd.prim[2], infl)) {
-          mine = true;
-          f3 vin;
-          if (nt > 0 && tape_b[nt - 1] == b) {
-            vin = tape_v[--nt];
-          } else {  // tape overflow: replay the contacts before b
-            vin = v0;
-            for (int b2 = 0; b2 < b; ++b2)
-              if ((near >> b2) & 1ull)
-                contact(vin, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b2), P.dt);
-          }
-          contact_adj(vin, gx, Bd, P.dt, mode, g, pg);
-        }
+      const bool mine = (hit >> b) & 1ull;
+      if (!__any(mine)) {  // uniform
+        if (DET && lane < 4)  // its partial for this block is zero
+          reinterpret_cast<float4 *>(D.body_partial + ((size_t)e * B.max_bodies + b) * 16)[lane] =
+              make_float4(0.f, 0.f, 0.f, 0.f);
+        continue;
       }
-      if (DET) {
+      PoseGrad pg = posegrad_zero();
+      if (mine) {
+        const Body Bd = load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b);
+        f3 vin;
+        --nt;  // contacts are popped in reverse order of the replay
+        if (nt < kTapeMax) {
+          vin = tape_v[nt];
+        } else {  // beyond the tape: replay the contacts before b
+          vin = v0;
+          for (int b2 = 0; b2 < b; ++b2)
+            if ((hit >> b2) & 1ull)
+              contact(vin, gx, load_body(P.prim, P.args, P.pos, P.rot, P.npos, P.nrot, b2), P.dt);
+        }
+        contact_adj(vin, gx, Bd, P.dt, mode, g, pg);
+      }
+      float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
+                        pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
+#pragma unroll
+      for (int q = 0; q < 14; ++q) vals[q] = wave_sum(vals[q]);
+      if (DET) {  // this block's partial, reduced in block order by k_det_body_reduce
         float4 *part = reinterpret_cast<float4 *>(D.body_partial + ((size_t)e * B.max_bodies + b) * 16);
-        if (__any(mine)) {
-          float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
-                            pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
-#pragma unroll
-          for (int q = 0; q < 14; ++q) vals[q] = wave_sum(vals[q]);
-          if (lane == 0) {
-            part[0] = make_float4(vals[0], vals[1], vals[2], vals[3]);
-            part[1] = make_float4(vals[4], vals[5], vals[6], vals[7]);
-            part[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
-            part[3] = make_float4(vals[12], vals[13], 0.f, 0.f);
-          }
-        } else if (lane < 4) {
-          part[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-      } else if (__any(mine)) {
-        float vals[14] = {pg.pos.x,  pg.pos.y,  pg.pos.z,  pg.rot.w,  pg.rot.x,  pg.rot.y,  pg.rot.z,
-                          pg.npos.x, pg.npos.y, pg.npos.z, pg.nrot.w, pg.nrot.x, pg.nrot.y, pg.nrot.z};
-#pragma unroll
-        for (int e = 0; e < 14; ++e) vals[e] = wave_sum(vals[e]);
         if (lane == 0) {
-          const ddm_state &cg = cur_grad_tab[si];
-          const ddm_state &ng = next_grad_tab[si];
-          float *cp = cg.body_pos + 3 * b, *cr = cg.body_rot + 4 * b;
-          float *np = ng.body_pos + 3 * b, *nr = ng.body_rot + 4 * b;
-          atomicAdd(cp + 0, vals[0]);
-          atomicAdd(cp + 1, vals[1]);
-          atomicAdd(cp + 2, vals[2]);
-          atomicAdd(cr + 0, vals[3]);
-          atomicAdd(cr + 1, vals[4]);
-          atomicAdd(cr + 2, vals[5]);
-          atomicAdd(cr + 3, vals[6]);
-          atomicAdd(np + 0, vals[7]);
-          atomicAdd(np + 1, vals[8]);
-          atomicAdd(np + 2, vals[9]);
-          atomicAdd(nr + 0, vals[10]);
-          atomicAdd(nr + 1, vals[11]);
-          atomicAdd(nr + 2, vals[12]);
-          atomicAdd(nr + 3, vals[13]);
+          part[0] = make_float4(vals[0], vals[1], vals[2], vals[3]);
+          part[1] = make_float4(vals[4], vals[5], vals[6], vals[7]);
+          part[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
+          part[3] = make_float4(vals[12], vals[13], 0.f, 0.f);
         }
+      } else if (lane == 0) {
+        const ddm_state &cg = cur_grad_tab[si];
+        const ddm_state &ng = next_grad_tab[si];
+        float *cp = cg.body_pos + 3 * b, *cr = cg.body_rot + 4 * b;
+        float *np = ng.body_pos + 3 * b, *nr = ng.body_rot + 4 * b;
+        atomicAdd(cp + 0, vals[0]);
+        atomicAdd(cp + 1, vals[1]);
+        atomicAdd(cp + 2, vals[2]);
+        atomicAdd(cr + 0, vals[3]);
+        atomicAdd(cr + 1, vals[4]);
+        atomicAdd(cr + 2, vals[5]);
+        atomicAdd(cr + 3, vals[6]);
+        atomicAdd(np + 0, vals[7]);
+        atomicAdd(np + 1, vals[8]);
+        atomicAdd(np + 2, vals[9]);
+        atomicAdd(nr + 0, vals[10]);
+        atomicAdd(nr + 1, vals[11]);
+        atomicAdd(nr + 2, vals[12]);
+        atomicAdd(nr + 3, vals[13]);
       }
     }
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);

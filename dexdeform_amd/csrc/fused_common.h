@@ -351,10 +351,30 @@ __device__ __forceinline__ NodeParams scene_node_params(const ddm_scene &S, cons
   return P;
 }
 
+// Index of the calling wave in its workgroup, as a value the compiler knows to be the same
+// on every lane (threadIdx.x >> 6 is, but divergence analysis cannot tell): list entries,
+// scene records and body parameters addressed through it are then read with scalar loads
+// into SGPRs instead of a vector load per lane.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// v combined with the lane that DPP control CTRL selects; rows (16 lanes) outside ROW_MASK
+// keep 0, so the sum below adds nothing there
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_or_zero(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+// Sum over the 64 lanes of a wave, the same value on every lane. Data-parallel-primitive
+// moves (VALU operations, no LDS round trip): lane ^ 1, lane ^ 2, the half-row and row
+// mirrors give every lane its row's sum; row_bcast:15 and row_bcast:31 carry rows 0 and
+// 0-1 up to row 3, whose lane 63 then holds the total. Fixed order: deterministic.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_or_zero<0xb1>(v);       // quad_perm [1, 0, 3, 2]
+  v += dpp_or_zero<0x4e>(v);       // quad_perm [2, 3, 0, 1]
+  v += dpp_or_zero<0x141>(v);      // row_half_mirror
+  v += dpp_or_zero<0x140>(v);      // row_mirror
+  v += dpp_or_zero<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_or_zero<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 struct NullScatter {

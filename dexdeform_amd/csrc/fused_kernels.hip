@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
 template <bool ZERO, bool SAVE, bool SKIP_RESTORED>
 __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                               const ddm_state *__restrict__ next_tab, int32_t *clear_count) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = wave_id(), lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
   const int n_listed = *B.block_count;

@@ -334,9 +334,9 @@ def test_persistent_loop_serves_requests_submitted_while_it_runs():
     srv = RolloutServer(device=_dev(), arena_blocks=16384, memory_budget=8 << 30).start()
     try:
         futs = []
-        for r in reqs:
+        for r in reqs:  # 0.5 ms apart: each request takes a few ms on the device
             futs.append(srv.submit_async(r))
-            _time.sleep(0.004)
+            _time.sleep(0.0005)
         dup = srv.submit_async(copy.copy(reqs[-1]))  # same id while the first is being served
         with pytest.raises(ValueError):
             srv.submit_async(Request(99, reqs[0].spec, reqs[0].x0, reqs[0].pose_pos[:1], reqs[0].pose_rot[:1],
