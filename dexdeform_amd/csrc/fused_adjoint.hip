@@ -15,6 +15,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
                                                   const ddm_state *__restrict__ next_tab,
                                                   const ddm_state *__restrict__ cur_grad_tab,
                                                   const ddm_state *__restrict__ next_grad_tab) {
+  __shared__ WaveBodies s_bodies[kWG / 64];
   const int wave = wave_id(), lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
@@ -33,6 +34,7 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
     const f3 mv{a.y, a.z, a.w};
     const bool act = node_active(m);
     NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
+    stage_bodies(P, s_bodies[wave], lane);
     const unsigned long long near = near_bodies(P, c.ix, c.iy, c.iz, lane);  // others: no contact here
     const f3 gx = node_position(c.ix, c.iy, c.iz, P.dx);
     const int mode = S.adjoint_mode;

@@ -351,6 +351,43 @@ __device__ __forceinline__ NodeParams scene_node_params(const ddm_scene &S, cons
   return P;
 }
 
+// Body parameters of one scene, staged in LDS for one wave: (type, friction, softness,
+// round), args, and the poses at the substep and the next, each as its own array as in
+// global memory (so load_body and near_bodies read them unchanged)
+constexpr int kBodyFloats = 22;
+struct WaveBodies {
+  float f[kBodyFloats * DDM_MAX_BODIES];
+};
+// Copies P's bodies into W (lane b copies body b) and points P at the copy. The grid
+// kernels then read a body's parameters with LDS loads in their loops over bodies instead
+// of a vector-memory round trip per body (they cannot be scalar loads: the compiler cannot
+// prove that the kernel's own stores leave them alone). Call with all 64 lanes of the
+// wave; LDS operations of one wave execute in order, so its later reads see the copy.
+__device__ __forceinline__ void stage_bodies(NodeParams &P, WaveBodies &W, int lane) {
+  float *prim = W.f, *args = W.f + 4 * DDM_MAX_BODIES, *pos = W.f + 8 * DDM_MAX_BODIES,
+        *rot = W.f + 11 * DDM_MAX_BODIES, *npos = W.f + 15 * DDM_MAX_BODIES, *nrot = W.f + 18 * DDM_MAX_BODIES;
+  if (lane < P.nb) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      prim[4 * lane + k] = P.prim[4 * lane + k];
+      args[4 * lane + k] = P.args[4 * lane + k];
+      rot[4 * lane + k] = P.rot[4 * lane + k];
+      nrot[4 * lane + k] = P.nrot[4 * lane + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      pos[3 * lane + k] = P.pos[3 * lane + k];
+      npos[3 * lane + k] = P.npos[3 * lane + k];
+    }
+  }
+  P.prim = prim;
+  P.args = args;
+  P.pos = pos;
+  P.rot = rot;
+  P.npos = npos;
+  P.nrot = nrot;
+}
+
 // Index of the calling wave in its workgroup, as a value the compiler knows to be the same
 // on every lane (threadIdx.x >> 6 is, but divergence analysis cannot tell): list entries,
 // scene records and body parameters addressed through it are then read with scalar loads

@@ -156,6 +156,7 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
 template <bool ZERO, bool SAVE, bool SKIP_RESTORED>
 __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                               const ddm_state *__restrict__ next_tab, int32_t *clear_count) {
+  __shared__ WaveBodies s_bodies[kWG / 64];
   const int wave = wave_id(), lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
@@ -170,7 +171,8 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
     float4 a = mmv[gi];
     f3 v{0.f, 0.f, 0.f};
-    const NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
+    NodeParams P = scene_node_params(S, cur_tab[si], next_tab[si]);
+    stage_bodies(P, s_bodies[wave], lane);
     const unsigned long long near = near_bodies(P, c.ix, c.iy, c.iz, lane);
     if (node_active(a.x)) v = node_fwd(a.x, f3{a.y, a.z, a.w}, c.ix, c.iy, c.iz, P, near);
     gv[gi] = make_float4(v.x, v.y, v.z, 0.f);
