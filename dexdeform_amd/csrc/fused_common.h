@@ -221,12 +221,20 @@ __device__ int build_groups(bool valid, int key, int n_valid, GroupScratch &sc, 
 // column shares each record read, the weight product w_x w_y and the x and y terms of the
 // affine product between its nodes; contrib(rc, slot, i, j, acc) adds one record's three
 // contributions with the same operations, in the same order, as per-node code would.
+// When the 9 G tasks leave lanes of the workgroup idle (small chunks: a few dozen groups),
+// each task's records are dealt to P parts (record q to part (q - first) mod P) on
+// neighbouring lanes, whose sums meet in sink's atomics: the longest task, which sets how
+// long a wave takes, is then about P times shorter. P = 1 when the tasks fill the lanes.
 template <int NC, class Contrib, class Sink>
 __device__ void group_column_tasks(int G, const Groups &gr, const StencilRecs &rc, const Tile *L, Contrib contrib,
                                    Sink sink) {
-  for (int task = (int)threadIdx.x; task < 9 * G; task += kWG) {
-    const int g = task / 9, ij = task - 9 * (task / 9);
+  const int P = G > 0 ? max(1, min(4, kWG / (9 * G))) : 1;  // uniform
+  for (int task = (int)threadIdx.x; task < 9 * G * P; task += kWG) {
+    const int gt = task / P, part = task - P * gt;
+    const int g = gt / 9, ij = gt - 9 * g;
     const int i = ij / 3, j = ij - 3 * (ij / 3);
+    const int q0 = gr.off[g] + part, qe = gr.off[g + 1];
+    if (q0 >= qe) continue;  // a part with no records
     int b[3];
     unpack_base(gr.key[g], b);
     if (L && !L->has_x(b[0] + i)) continue;
@@ -235,7 +243,7 @@ __device__ void group_column_tasks(int G, const Groups &gr, const StencilRecs &r
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int c = 0; c < NC; ++c) acc[k][c] = 0.f;
-    for (int q = gr.off[g], qe = gr.off[g + 1]; q < qe; ++q) contrib(rc, q, i, j, acc);
+    for (int q = q0; q < qe; q += P) contrib(rc, q, i, j, acc);
 #pragma unroll
     for (int k = 0; k < 3; ++k) sink(b[0] + i, b[1] + j, b[2] + k, acc[k]);
   }
@@ -248,15 +256,29 @@ union ScatterLDS {
   GroupScratch grp;
 };
 
+// v from the lane that DPP control CTRL selects; identity on rows outside ROW_MASK
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_or(int v, int identity) {
+  return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROW_MASK, 0xf, false);
+}
+// minimum / maximum over the wave, the same value on every lane (the DPP steps of wave_sum)
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
+  v = min(v, dpp_or<0xb1>(v, INT_MAX));
+  v = min(v, dpp_or<0x4e>(v, INT_MAX));
+  v = min(v, dpp_or<0x141>(v, INT_MAX));
+  v = min(v, dpp_or<0x140>(v, INT_MAX));
+  v = min(v, dpp_or<0x142, 0xa>(v, INT_MAX));
+  v = min(v, dpp_or<0x143, 0xc>(v, INT_MAX));
+  return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
+  v = max(v, dpp_or<0xb1>(v, INT_MIN));
+  v = max(v, dpp_or<0x4e>(v, INT_MIN));
+  v = max(v, dpp_or<0x141>(v, INT_MIN));
+  v = max(v, dpp_or<0x140>(v, INT_MIN));
+  v = max(v, dpp_or<0x142, 0xa>(v, INT_MIN));
+  v = max(v, dpp_or<0x143, 0xc>(v, INT_MIN));
+  return __builtin_amdgcn_readlane(v, 63);
 }
 
 // bounding box of the chunk's stencil bases -> tile covering all stencil nodes.
