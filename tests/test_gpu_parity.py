@@ -170,51 +170,69 @@ def test_fused_backward_one_substep(plain, mode):
             assert e <= 1e-3, (name, e)
 
 
-def test_gradient_overwrite_equals_accumulate_from_zero(plain):
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_gradient_overwrite_equals_accumulate_from_zero(plain, deterministic):
     """DDM_GRAD_OVERWRITE: the adjoint substep writes the current state's x/v/F/C
     gradients without reading them (a NaN-filled buffer comes out finite), and the
     result equals accumulation into zeros; accumulation into random values R adds the
-    same gradient to R. Body-pose gradients are accumulated in both modes."""
+    same gradient to R. Body-pose gradients are accumulated in both modes.
+    Deterministic mode: every run sums in the same order, so overwrite and accumulation
+    from zero agree bit for bit, and (R + g) - R differs from g only by the rounding of
+    the addition. Atomic mode: two runs differ by the order of FP32 grid atomics, which
+    the SVD adjoint amplifies by 1 / (sigma_j^2 - sigma_i^2) for F and C; the bar is
+    then the larger of 1e-4 and three times the difference between two accumulations
+    from zero (the run-to-run noise of this very comparison), on particles with
+    well-separated singular values for F and C."""
     sc = plain["sc"]
     osim = O.OracleSim(oracle_scene(sc), np.float32)
     st = plain["states"][20]
     nx = osim.substep(st, plain["P"][21], plain["R"][21])
     g_next = _random_grads(sc.n, sc.nb)
-    eng = _engine(sc, 0)
+    sc.spec.adjoint_mode = 0
+    eng = Engine([sc.spec], device=_dev(), with_grad=True, deterministic=deterministic)
     cur, nxt, gnxt = eng.new_state(), eng.new_state(), eng.new_grad()
     _to_buf(st, cur)
     _to_buf(nx, nxt)
-    _to_buf(O.State(g_next.x, g_next.v, g_next.F, g_next.C, g_next.body_pos, g_next.body_rot), gnxt)
-    g0, gow, gr = eng.new_grad(), eng.new_grad(), eng.new_grad()
+    g0, g0b, gow, gr = eng.new_grad(), eng.new_grad(), eng.new_grad(), eng.new_grad()
     gen = torch.Generator(device="cpu").manual_seed(5)
     R = [torch.randn(t.shape, generator=gen).to(t.device) for t in (gr.x, gr.v, gr.F, gr.C)]
     for t, r in zip((gr.x, gr.v, gr.F, gr.C), R):
         t.copy_(r)
     for t in (gow.x, gow.v, gow.F, gow.C):
         t.fill_(float("nan"))
-    for g, ow in ((g0, False), (gow, True), (gr, False)):
+    for g, ow in ((g0, False), (g0b, False), (gow, True), (gr, False)):
         # the next state's body-pose gradients are accumulated into: same seed every call
         _to_buf(O.State(g_next.x, g_next.v, g_next.F, g_next.C, g_next.body_pos, g_next.body_rot), gnxt)
         eng.backward(cur, nxt, g, gnxt, overwrite=ow)
     torch.cuda.synchronize()
-    # two adjoint runs differ by the order of FP32 grid atomics, which the SVD adjoint
-    # amplifies by 1 / (sigma_j^2 - sigma_i^2) for F and C: compare those on particles
-    # with well-separated singular values
     Ft = np.einsum("nij,njk->nik", np.eye(3)[None] + sc.spec.dt * st.C.astype(np.float64), st.F.astype(np.float64))
     s = np.linalg.svd(Ft, compute_uv=False)
     well = np.min(np.abs(np.diff(s * s, axis=1)), axis=1) > 1e-3
     assert well.sum() >= 10
-    for name, a, b, c, r in zip("xvFC", _np(g0), _np(gow), _np(gr), R):
+    for name, a, a2, b, c, r in zip("xvFC", _np(g0), _np(g0b), _np(gow), _np(gr), R):
         assert np.isfinite(b).all(), name
         r = r.cpu().numpy()
+        if deterministic:
+            assert np.array_equal(b, a) and np.array_equal(a2, a), name
+            e_acc = np.abs((c - r) - a).max() / max(np.abs(r).max(), np.abs(a).max())
+            print(f"{name}: overwrite == accumulate-from-zero; accumulate into R minus R {e_acc:.1e}")
+            assert e_acc <= 1e-6, name  # the rounding of r + g, relative to max |r|
+            continue
         sel = well if name in "FC" else slice(None)
+        scale = max(np.abs(r).max(), np.abs(a[sel]).max())
+        noise_ow, noise_acc = rel_err(a2[sel], a[sel]), np.abs(a2 - a)[sel].max() / scale
         e_ow = rel_err(b[sel], a[sel])
-        e_acc = np.abs((c - r) - a)[sel].max() / max(np.abs(r).max(), np.abs(a[sel]).max())
-        print(f"{name}: overwrite vs accumulate-from-zero {e_ow:.1e}; accumulate into R minus R {e_acc:.1e}")
-        assert e_ow <= 1e-4 and e_acc <= 1e-4, name
+        e_acc = np.abs((c - r) - a)[sel].max() / scale
+        print(f"{name}: overwrite vs accumulate-from-zero {e_ow:.1e} (two accumulations {noise_ow:.1e}); "
+              f"accumulate into R minus R {e_acc:.1e} (two accumulations {noise_acc:.1e})")
+        assert e_ow <= max(1e-4, 3 * noise_ow) and e_acc <= max(1e-4, 3 * noise_acc), name
     for name, a, b in (("body_pos", g0.body_pos, gow.body_pos), ("body_rot", g0.body_rot, gow.body_rot)):
-        e = rel_err(b.cpu().numpy(), a.cpu().numpy())
-        assert e <= 1e-4, (name, e)
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        if deterministic:
+            assert np.array_equal(b, a), name
+        else:
+            e = rel_err(b, a)
+            assert e <= 1e-4, (name, e)
 
 
 @pytest.mark.parametrize("layout", ["slabs", "global"])
