@@ -85,7 +85,9 @@ class Rollout:
         grid of the forward pass, so the adjoint substeps skip recomputing P2G and the
         grid update (44 B per particle plus 2 KiB per active grid block per substep).
         tape_blocks bounds the blocks one scene's snapshot holds per substep (an int for
-        every scene or a list per scene; default: Engine.tape_capacities); a scene whose substep lists more is recomputed by its
+        every scene or a list per scene; default: every scene's whole grid when the tape's
+        snapshots then take at most a quarter of the device's memory, else
+        Engine.tape_capacities); a scene whose substep lists more is recomputed by its
         adjoint instead (decided on the device)."""
         self.engine = engine
         self.T = int(T)
@@ -122,7 +124,12 @@ class Rollout:
         self.tape = None
         if tape:
             if tape_blocks is None:
-                caps = engine.tape_capacities()
+                # whole grids (no substep can overflow, so the adjoint launches no recompute
+                # kernels) when the K slots of them take at most a quarter of the device
+                whole = engine.tape_capacities(whole_grid=True)
+                budget = torch.cuda.get_device_properties(engine.device).total_memory // 4
+                fits = self.K * sum(whole) * engine.TAPE_BLOCK_BYTES <= budget
+                caps = whole if fits else engine.tape_capacities()
             elif isinstance(tape_blocks, (list, tuple)):
                 caps = [int(c) for c in tape_blocks]
             else:
