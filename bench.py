@@ -404,8 +404,9 @@ class KernelTimer:
         taped = ro.tape is not None
         fwd_names, bwd_names = (FWD_NAMES_TAPED, BWD_NAMES_TAPED) if taped else (FWD_NAMES, BWD_NAMES)
         batches = [eng._batch_with_scenes(ro.tape.scene_tabs[t]) for t in range(ro.T)] if taped else None
-        # the adjoint substeps write the state-t gradients (as Rollout.backward does)
-        ow = _lib.DDM_GRAD_OVERWRITE
+        # the adjoint substeps write the state-t gradients, and with a complete tape skip
+        # the recompute kernels (as Rollout.backward does)
+        ow = _lib.DDM_GRAD_OVERWRITE | (_lib.DDM_TAPE_COMPLETE if taped and ro.tape_complete else 0)
         bwd_batches = ([eng._batch_with(ro.tape.scene_tabs[t], ow) for t in range(ro.T)] if taped else
                        [eng._batch_with(None, ow)] * ro.T)
         for t in range(ro.T):
