@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
             V = quat_mat(ldq(S.tmp_qv + 4 * (size_t)p));
   f3 sig = ld3(S.tmp_sig + p3);
   P2GAdj a;
-  p2g_particle_adj(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+  p2g_particle_adj<true>(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                    ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
   m33 gFt, gF, gC;
   asm volatile("" : "+v"(Fp));  // F is read again (from the cache) rather than held across the P2G adjoint
