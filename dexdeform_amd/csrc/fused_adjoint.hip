@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64) void k_det_body_reduce(ddm_batch B, ddm_det D,
 }
 
 // adjoint of G2P: adds to the x adjoint and scatters the node-velocity adjoint
-__global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+__global__ __launch_bounds__(kWG, 5) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ next_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
@@ -206,11 +206,10 @@ __global__ __launch_bounds__(kWG) void k_g2p_adj(ddm_batch B, const ddm_state *_
   int key = 0;
   if (valid) {
     BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
-    NullScatter none;
     gC = ld9(next_grad.C + 9 * (size_t)p);
-    f3 gxp = g2p_particle_adj(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
-                              ld3(next_grad.x + 3 * (size_t)p), ld3(next_grad.v + 3 * (size_t)p), gC, gv, none,
-                              S.inv_dx, S.dx, S.dt, S.ground_height, dim, &g_nv, &st);
+    f3 gxp = g2p_particle_adj_gather(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
+                                     ld3(next_grad.x + 3 * (size_t)p), ld3(next_grad.v + 3 * (size_t)p), gC, gv,
+                                     S.inv_dx, S.dx, S.dt, S.ground_height, dim, g_nv, st);
     if (B.options & DDM_GRAD_OVERWRITE) st3(cur_grad.x + 3 * (size_t)p, gxp);
     else add3(cur_grad.x + 3 * (size_t)p, gxp);
     key = pack_base(st.base);
