@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
                    ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
   m33 gFt, gF, gC;
   asm volatile("" : "+v"(Fp));  // F is read again (from the cache) rather than held across the P2G adjoint
-  svd_adj(ld9(Fp), C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
+  svd_adj<true>(ld9(Fp), C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
   const bool overwrite = B.options & DDM_GRAD_OVERWRITE;
   add3(cur_grad.x + p3, a.x);  // onto the G2P adjoint's contribution of this substep
   if (overwrite) st3(cur_grad.v + p3, a.v);
