@@ -181,8 +181,12 @@ __global__ __launch_bounds__(64) void k_det_body_reduce(ddm_batch B, ddm_det D,
   }
 }
 
-// adjoint of G2P: adds to the x adjoint and scatters the node-velocity adjoint
-__global__ __launch_bounds__(kWG, 5) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+// adjoint of G2P: adds to the x adjoint and scatters the node-velocity adjoint. LAT: the
+// variant for launches of at most kLatencyChunks chunks (about one wave per SIMD), where
+// the kernel runs at one chunk's latency: the node velocities are gathered in one round
+// trip (more registers, which such a launch has to spare).
+template <bool LAT>
+__global__ __launch_bounds__(kWG, LAT ? 1 : 5) void k_g2p_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ next_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(kWG, 5) void k_g2p_adj(ddm_batch B, const ddm_state
   if (valid) {
     BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
     gC = ld9(next_grad.C + 9 * (size_t)p);
-    f3 gxp = g2p_particle_adj_gather(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
+    f3 gxp = g2p_particle_adj_gather<LAT>(ld3(cur.x + 3 * (size_t)p), ld3(next.v + 3 * (size_t)p),
                                      ld3(next_grad.x + 3 * (size_t)p), ld3(next_grad.v + 3 * (size_t)p), gC, gv,
                                      S.inv_dx, S.dx, S.dt, S.ground_height, dim, g_nv, st);
     if (B.options & DDM_GRAD_OVERWRITE) st3(cur_grad.x + 3 * (size_t)p, gxp);
@@ -344,8 +348,10 @@ __global__ __launch_bounds__(kWG) void k_det_gather_g2p_adj(ddm_batch B, ddm_det
 }
 
 // adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
-// at most 168 VGPRs (3 waves per SIMD instead of 2, with a few bytes of spill): 11 % faster
-__global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+// at most 168 VGPRs (3 waves per SIMD instead of 2): 11 % faster. LAT: as k_g2p_adj's, the
+// node gradients are gathered first, in one round trip that overlaps the particle's stress.
+template <bool LAT>
+__global__ __launch_bounds__(kWG, LAT ? 1 : 3) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
                                                  const ddm_state *__restrict__ next_grad_tab) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
@@ -363,15 +369,18 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
   const m33 C = ld9(cur.C + p9);
   // F_tmp as the forward P2G computed it (advect_F is evaluated without contraction here
   // too, so it is the same bits); its SVD from the temporaries (U and V as quaternions)
-  const m33 Ft = advect_F(ld9(Fp), C, S.dt), U = quat_mat(ldq(S.tmp_qu + 4 * (size_t)p)),
+  const m33 F0 = ld9(Fp);
+  const m33 Ft = advect_F(F0, C, S.dt), U = quat_mat(ldq(S.tmp_qu + 4 * (size_t)p)),
             V = quat_mat(ldq(S.tmp_qv + 4 * (size_t)p));
   f3 sig = ld3(S.tmp_sig + p3);
   P2GAdj a;
-  p2g_particle_adj<true>(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
+  p2g_particle_adj<true, LAT>(ld3(cur.x + p3), ld3(cur.v + p3), C, Ft, U, sig, V, S.mass[p], S.vol[p],
                    ld3(S.mu_lam_yield + p3), ld9(next_grad.F + p9), gat, S.dx, S.inv_dx, S.dt, dim, a);
   m33 gFt, gF, gC;
-  asm volatile("" : "+v"(Fp));  // F is read again (from the cache) rather than held across the P2G adjoint
-  svd_adj<true>(ld9(Fp), C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
+  // F is read again (from the cache) rather than held across the P2G adjoint, except in
+  // the latency variant, which has the registers and would wait for the read
+  if (!LAT) asm volatile("" : "+v"(Fp));
+  svd_adj<true>(LAT ? F0 : ld9(Fp), C, U, V, sig, a.Ft, a.U, a.V, a.sig, S.dt, gFt, gF, gC);
   const bool overwrite = B.options & DDM_GRAD_OVERWRITE;
   add3(cur_grad.x + p3, a.x);  // onto the G2P adjoint's contribution of this substep
   if (overwrite) st3(cur_grad.v + p3, a.v);
@@ -387,7 +396,8 @@ __global__ __launch_bounds__(kWG, 3) void k_p2g_adj(ddm_batch B, const ddm_state
 
 void launch_g2p_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                     const ddm_state *next_grad, hipStream_t stream) {
-  k_g2p_adj<<<b.n_chunks, kWG, 0, stream>>>(b, cur, next, cur_grad, next_grad);
+  if (b.n_chunks <= kLatencyChunks) k_g2p_adj<true><<<b.n_chunks, kWG, 0, stream>>>(b, cur, next, cur_grad, next_grad);
+  else k_g2p_adj<false><<<b.n_chunks, kWG, 0, stream>>>(b, cur, next, cur_grad, next_grad);
 }
 void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                      const ddm_state *next_grad, hipStream_t stream) {
@@ -405,7 +415,8 @@ void launch_grid_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *
 }
 void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
                     hipStream_t stream) {
-  k_p2g_adj<<<b.n_chunks, kWG, 0, stream>>>(b, cur, cur_grad, next_grad);
+  if (b.n_chunks <= kLatencyChunks) k_p2g_adj<true><<<b.n_chunks, kWG, 0, stream>>>(b, cur, cur_grad, next_grad);
+  else k_p2g_adj<false><<<b.n_chunks, kWG, 0, stream>>>(b, cur, cur_grad, next_grad);
 }
 
 }  // namespace ddm_fused

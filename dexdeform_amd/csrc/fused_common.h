@@ -42,6 +42,11 @@ constexpr int kTileP2G = 704;
 constexpr int kTileG2P = 960;
 constexpr int kHash = 512;       // open-addressing table of the chunk's stencil bases
 constexpr int kTapeMax = 8;      // contact tape entries kept per node in the adjoint
+// Particle launches of at most this many chunks (256-thread workgroups) put at most one
+// workgroup on each of the 256 CUs, one wave per SIMD: they run at one chunk's latency,
+// not at a throughput bound, and use the latency variants of the gathering kernels (loads
+// issued together; as many registers as one wave per SIMD allows).
+constexpr int kLatencyChunks = 256;
 
 __device__ __forceinline__ int node_index(const ddm_scene &S, int x, int y, int z) {
   int nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
@@ -383,23 +388,39 @@ struct WaveBodies {
 // Copies P's bodies into W (lane b copies body b) and points P at the copy. The grid
 // kernels then read a body's parameters with LDS loads in their loops over bodies instead
 // of a vector-memory round trip per body (they cannot be scalar loads: the compiler cannot
-// prove that the kernel's own stores leave them alone). Call with all 64 lanes of the
-// wave; LDS operations of one wave execute in order, so its later reads see the copy.
+// prove that the kernel's own stores leave them alone). All 22 loads are issued before
+// the first LDS store: written as load-then-store pairs, each load would wait for the
+// store before it (the source pointers are generic and might alias the copy), 22
+// dependent memory round trips per staging. Call with all 64 lanes of the wave; LDS
+// operations of one wave execute in order, so its later reads see the copy.
 __device__ __forceinline__ void stage_bodies(NodeParams &P, WaveBodies &W, int lane) {
   float *prim = W.f, *args = W.f + 4 * DDM_MAX_BODIES, *pos = W.f + 8 * DDM_MAX_BODIES,
         *rot = W.f + 11 * DDM_MAX_BODIES, *npos = W.f + 15 * DDM_MAX_BODIES, *nrot = W.f + 18 * DDM_MAX_BODIES;
   if (lane < P.nb) {
+    float r[kBodyFloats];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      prim[4 * lane + k] = P.prim[4 * lane + k];
-      args[4 * lane + k] = P.args[4 * lane + k];
-      rot[4 * lane + k] = P.rot[4 * lane + k];
-      nrot[4 * lane + k] = P.nrot[4 * lane + k];
+      r[k] = P.prim[4 * lane + k];
+      r[4 + k] = P.args[4 * lane + k];
+      r[8 + k] = P.rot[4 * lane + k];
+      r[12 + k] = P.nrot[4 * lane + k];
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      pos[3 * lane + k] = P.pos[3 * lane + k];
-      npos[3 * lane + k] = P.npos[3 * lane + k];
+      r[16 + k] = P.pos[3 * lane + k];
+      r[19 + k] = P.npos[3 * lane + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      prim[4 * lane + k] = r[k];
+      args[4 * lane + k] = r[4 + k];
+      rot[4 * lane + k] = r[8 + k];
+      nrot[4 * lane + k] = r[12 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      pos[3 * lane + k] = r[16 + k];
+      npos[3 * lane + k] = r[19 + k];
     }
   }
   P.prim = prim;

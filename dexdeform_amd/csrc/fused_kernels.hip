@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kWG) void k_g2p(ddm_batch B, const ddm_state *__res
   BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
   f3 xo, vo;
   m33 Co;
-  g2p_particle(ld3(cur.x + 3 * (size_t)p), gv, S.inv_dx, S.dx, S.dt, S.ground_height, dim, xo, vo, Co);
+  g2p_particle_prefetch(ld3(cur.x + 3 * (size_t)p), gv, S.inv_dx, S.dx, S.dt, S.ground_height, dim, xo, vo, Co);
   st3(next.x + 3 * (size_t)p, xo);
   st3(next.v + 3 * (size_t)p, vo);
   st9(next.C + 9 * (size_t)p, Co);
