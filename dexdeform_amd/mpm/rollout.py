@@ -97,6 +97,7 @@ class Rollout:
         if self.K <= 0 or self.T % self.K:
             raise ValueError("checkpoint_every must divide the horizon")
         self.full = self.K == self.T
+        self._last_segment_taped = False  # forward() ran the last segment taped
         nb_rows = max(NB, 1)
         self.pose_pos = torch.zeros(self.T + 1, nb_rows, 3, device=dev)
         self.pose_rot = torch.zeros(self.T + 1, nb_rows, 4, device=dev)
@@ -174,12 +175,16 @@ class Rollout:
             return self.seg[self.T]
         # checkpointed: run each segment in the segment buffer and keep its end state;
         # segment 0 restarts from checkpoint 0 too, since a previous pass left a later
-        # segment's start in the buffer
-        for c in range(self.T // self.K):
+        # segment's start in the buffer. The last segment runs taped when there is a tape:
+        # the backward pass starts with it and need not run it again.
+        n_seg = self.T // self.K
+        for c in range(n_seg):
             self.seg[0].copy_(self.ckpt[c])
+            last = c == n_seg - 1 and self.tape is not None
             for j in range(self.K):
-                self._fwd(j, c * self.K + j)
+                self._fwd(j, c * self.K + j, taped=last)
             self.ckpt[c + 1].copy_(self.seg[self.K])
+        self._last_segment_taped = self.tape is not None
         self._final = self.ckpt[-1]
         return self._final
 
@@ -197,10 +202,13 @@ class Rollout:
         n_seg = self.T // self.K
         for c in range(n_seg - 1, -1, -1):
             base = c * self.K
-            if not self.full:
+            # the segment buffer and the tape still hold the last segment from forward()
+            # until an earlier segment is run again into them
+            if not self.full and not (c == n_seg - 1 and self._last_segment_taped):
                 self.seg[0].copy_(self.ckpt[c])
                 for j in range(self.K):
                     self._fwd(j, base + j, taped=self.tape is not None)
+                self._last_segment_taped = False
             for j in range(self.K - 1, -1, -1):
                 t = base + j
                 nxt_slot = cur_slot

@@ -325,6 +325,30 @@ def test_taped_adjoint_matches_recompute(K):
     assert e[0] <= 1e-4 and e[1] <= 1e-4
 
 
+def test_second_backward_after_one_checkpointed_taped_forward():
+    """forward() runs the last segment taped and backward() starts from that tape; a
+    second backward() without a forward pass runs that segment again (the first one
+    overwrote the tape with earlier segments) and gives the same gradients."""
+    dev = _dev()
+    scenes = [_scene(3000, 48, seed=2), _scene(6000, 64, seed=3, nb=5)]
+    eng = Engine([s.spec for s in scenes], device=dev)
+    ro = Rollout(eng, 20, checkpoint_every=5, tape=True)
+    ro.set_initial(torch.tensor(np.concatenate([s.x0 for s in scenes]), device=dev))
+    P = [_poses(s, 20, 100 + i) for i, s in enumerate(scenes)]
+    ro.set_poses(torch.tensor(np.concatenate([p[0] for p in P], 1), device=dev),
+                 torch.tensor(np.concatenate([p[1] for p in P], 1), device=dev))
+    xT = ro.forward().x.clone()
+    gx = torch.ones_like(xT) / len(xT)
+    out = []
+    for _ in range(2):
+        ro.backward(x_grad=gx)
+        torch.cuda.synchronize()
+        out.append((ro.pos_grad.cpu().numpy(), ro.initial_grad().x.cpu().numpy()))
+    e = [rel_err(out[1][0], out[0][0]), rel_err(out[1][1], out[0][1])]
+    print(f"second backward vs first: pose grad {e[0]:.2e} x0 grad {e[1]:.2e}")
+    assert e[0] <= 1e-4 and e[1] <= 1e-4
+
+
 def test_persistent_loop_serves_requests_submitted_while_it_runs():
     """The serving loop runs in a background thread; requests are submitted from this
     thread one by one while earlier ones are on the GPU, each future resolves with the
