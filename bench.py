@@ -48,10 +48,10 @@ KERNEL_BYTES = {
 # kernel changes; reported as roofline.traffic for the default workload only.
 TRAFFIC_BYTES = {
     "p2g_fwd": 220, "grid_fwd": 10, "g2p_fwd": 75, "p2g_recompute": 229, "grid_recompute": 7, "g2p_adj": 148,
-    "grid_adj": 18, "p2g_adj": 333,
+    "grid_adj": 18, "p2g_adj": 332,
     # taped rollout
     "taped_p2g_fwd": 266, "taped_grid_fwd": 17, "taped_g2p_fwd": 75, "taped_restore": 13, "taped_g2p_adj": 148,
-    "taped_grid_adj": 18, "taped_p2g_adj": 333,
+    "taped_grid_adj": 18, "taped_p2g_adj": 332,
 }
 FWD_NAMES = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES = ["p2g_recompute", "grid_recompute", "g2p_adj", "grid_adj", "p2g_adj"]
@@ -66,8 +66,8 @@ KERNEL_BYTES_TAPED = dict(KERNEL_BYTES, restore=0)
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 # VALU wave-instructions per wave (64 particles) and the VALUBusy fraction per kernel,
 # from rocprofv3 SQ_INSTS_VALU / SQ_WAVES and SQ_ACTIVE_INST_VALU on the default workload
-VALU_INSTR_PER_WAVE = {"p2g_fwd": 4047, "g2p_fwd": 1265, "g2p_adj": 2959, "p2g_adj": 2145}
-VALU_BUSY = {"p2g_fwd": 0.60, "g2p_fwd": 0.94, "g2p_adj": 0.53, "p2g_adj": 0.58}
+VALU_INSTR_PER_WAVE = {"p2g_fwd": 4047, "g2p_fwd": 1259, "g2p_adj": 2959, "p2g_adj": 2147}
+VALU_BUSY = {"p2g_fwd": 0.60, "g2p_fwd": 0.92, "g2p_adj": 0.52, "p2g_adj": 0.58}
 FWD_NAMES_TAPED = ["p2g_fwd", "grid_fwd", "g2p_fwd"]
 BWD_NAMES_TAPED = ["restore", "g2p_adj", "grid_adj", "p2g_adj"]
 
