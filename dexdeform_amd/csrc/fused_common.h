@@ -47,6 +47,12 @@ constexpr int kTapeMax = 8;      // contact tape entries kept per node in the ad
 // not at a throughput bound, and use the latency variants of the gathering kernels (loads
 // issued together; as many registers as one wave per SIMD allows).
 constexpr int kLatencyChunks = 256;
+// Forward launches of at most kLatencyChunks chunks over at most this many node blocks
+// skip the block list: P2G marks its blocks with plain stores of scene + 1 and the grid
+// update scans every block's flag (one wave per block). The list append costs P2G two
+// dependent device-scope atomics at its end (after its own tile flush has drained),
+// which such a launch waits for; the scan costs the grid update one flag load per wave.
+constexpr int kScanBlocks = 16384;
 
 __device__ __forceinline__ int node_index(const ddm_scene &S, int x, int y, int z) {
   int nby = S.dim[1] >> 2, nbz = S.dim[2] >> 2;
@@ -339,6 +345,15 @@ __device__ void flag_tile_blocks(const ddm_batch &B, const ddm_scene &S, int si,
     int bx = i / (nb[1] * nb[2]), by = (i / nb[2]) % nb[1], bz = i % nb[2];
     mark_block(B, block_of(S, b0[0] + bx, b0[1] + by, b0[2] + bz), si);
   }
+}
+// mark the (at most 8) blocks of one stencil with scene + 1, plain stores, no list (for
+// grid kernels that scan the flags)
+__device__ void store_stencil_flags(const ddm_batch &B, const ddm_scene &S, int si, const int base[3]) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c)
+        B.block_flags[block_of(S, (base[0] + 2 * a) >> 2, (base[1] + 2 * b) >> 2, (base[2] + 2 * c) >> 2)] =
+            (uint32_t)(si + 1);
 }
 // mark the (at most 8) blocks of one stencil
 __device__ void flag_stencil_blocks(const ddm_batch &B, const ddm_scene &S, int si, const int base[3]) {

@@ -27,7 +27,7 @@ __device__ __forceinline__ void tape_reset(const ddm_batch &B) {
 // at most 96 VGPRs (a few bytes of spill): five waves per SIMD, as the LDS allows
 template <int MODE>
 __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                             const ddm_state *__restrict__ next_tab) {
+                                             const ddm_state *__restrict__ next_tab, int scan) {
   __shared__ ScatterLDS<4, kTileP2G> sh;
   __shared__ StencilRecs rc;
   __shared__ Groups gr;
@@ -124,7 +124,8 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
         }
       }
     }
-    flag_tile_blocks(B, S, ch.x, T);
+    if (scan) flag_tile_blocks_det(B, S, ch.x, T);  // uniform; the same plain stores of scene + 1
+    else flag_tile_blocks(B, S, ch.x, T);
   } else {
     __syncthreads();  // publishes the staged records
     group_column_tasks<4>(G, gr, rc, nullptr, contrib, [&](int X, int Y, int Z, const float *a) {
@@ -137,7 +138,8 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
     for (int g = threadIdx.x; g < G; g += kWG) {
       int b[3];
       unpack_base(gr.key[g], b);
-      flag_stencil_blocks(B, S, ch.x, b);
+      if (scan) store_stencil_flags(B, S, ch.x, b);
+      else flag_stencil_blocks(B, S, ch.x, b);
     }
   }
   // at the end, where few registers are live (at the start it costs the kernel spills)
@@ -153,18 +155,30 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
 //   snapshot was restored.
 // clear_count (deterministic mode, forward): the bin counts of the listed blocks are cleared
 // with the grid (the sorted gather was their last reader).
-template <bool ZERO, bool SAVE, bool SKIP_RESTORED>
+// SCAN=true (forward, small launches): no block list; every block of the launch is
+//   visited and those whose flag P2G set (to scene + 1) are updated (ZERO only).
+template <bool ZERO, bool SAVE, bool SKIP_RESTORED, bool SCAN = false>
 __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                               const ddm_state *__restrict__ next_tab, int32_t *clear_count) {
+  static_assert(ZERO || !SCAN, "the scan clears the flags it reads");
   __shared__ WaveBodies s_bodies[kWG / 64];
   const int wave = wave_id(), lane = threadIdx.x & 63;
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *gv = reinterpret_cast<float4 *>(B.grid_v);
-  const int n_listed = *B.block_count;
+  const int n_items = SCAN ? B.n_blocks : *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
-  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
-    const int2 entry = list[e];
-    const int blk = entry.x, si = entry.y;
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_items; e += gridDim.x * (kWG / 64)) {
+    int blk, si;
+    if (SCAN) {
+      const uint32_t f = B.block_flags[e];
+      if (f == 0u) continue;  // the same on every lane
+      blk = e;
+      si = (int)f - 1;
+    } else {
+      const int2 entry = list[e];
+      blk = entry.x;
+      si = entry.y;
+    }
     const ddm_scene &S = B.scenes[si];
     if (SKIP_RESTORED && S.tape_mmv && S.tape_status[1] == 0) continue;
     BlockCoord c = block_node(S, blk, lane);
@@ -561,6 +575,13 @@ using namespace ddm_fused;
 
 #define DDM_API extern "C" __attribute__((visibility("default")))
 
+// forward launches that skip the block list (non-deterministic, few chunks, few blocks)
+static inline bool scan_blocks(const ddm_batch *b) {
+  return !(b->options & DDM_DETERMINISTIC) && b->n_chunks <= kLatencyChunks && b->n_blocks > 0 &&
+         b->n_blocks <= kScanBlocks;
+}
+static inline int scan_wgs(const ddm_batch *b) { return (b->n_blocks + kWG / 64 - 1) / (kWG / 64); }
+
 static inline void mark(hipEvent_t *ev, int i, hipStream_t stream) {
   if (ev) (void)hipEventRecord(ev[i], stream);
 }
@@ -570,12 +591,13 @@ static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_st
   int e = check(b, cur, next);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
-  const bool det = b->options & DDM_DETERMINISTIC;
+  const bool det = b->options & DDM_DETERMINISTIC, scan = scan_blocks(b);
   mark(ev, 0, stream);
   if (det) launch_det_p2g<kForward>(b, cur, next, stream);
-  else k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  else k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, scan);
   mark(ev, 1, stream);
-  k_grid<true, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
+  if (scan) k_grid<true, false, false, true><<<scan_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
+  else k_grid<true, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
@@ -608,7 +630,7 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
   if (b->options & DDM_DETERMINISTIC) launch_det_p2g<kRecompute>(b, cur, next, stream);
-  else k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  else k_p2g<kRecompute><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, 0);
   mark(ev, 1, stream);
   k_grid<false, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   mark(ev, 2, stream);
@@ -621,12 +643,13 @@ static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const 
   int e = check(b, cur, next);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
-  const bool det = b->options & DDM_DETERMINISTIC;
+  const bool det = b->options & DDM_DETERMINISTIC, scan = scan_blocks(b);
   mark(ev, 0, stream);
   if (det) launch_det_p2g<kForwardTaped>(b, cur, next, stream);
-  else k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  else k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, scan);
   mark(ev, 1, stream);
-  k_grid<true, true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
+  if (scan) k_grid<true, true, false, true><<<scan_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
+  else k_grid<true, true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
   mark(ev, 2, stream);
   k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
@@ -647,7 +670,7 @@ static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const
   // order that the sorted adjoint gathers read, so it always runs.
   if (!(b->options & DDM_TAPE_COMPLETE) || (b->options & DDM_DETERMINISTIC)) {
     if (b->options & DDM_DETERMINISTIC) launch_det_p2g<kRecomputeUntaped>(b, cur, next, stream);
-    else k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+    else k_p2g<kRecomputeUntaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, 0);
     k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   }
   mark(ev, 1, stream);
