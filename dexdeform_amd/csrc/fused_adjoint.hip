@@ -350,11 +350,23 @@ __global__ __launch_bounds__(kWG) void k_det_gather_g2p_adj(ddm_batch B, ddm_det
 // adjoint of P2G + SVD adjoint + chain through F_tmp = (I + dt C) F
 // at most 168 VGPRs (3 waves per SIMD instead of 2): 11 % faster. LAT: as k_g2p_adj's, the
 // node gradients are gathered first, in one round trip that overlaps the particle's stress.
+// restore_prev (taped adjoint, optional): kRestoreWGs workgroups per scene after the chunks'
+// put back the snapshots of the substep before, from its scene table (the arena blocks,
+// list and count they write are not read here; the grid adjoint before has finished).
 template <bool LAT>
 __global__ __launch_bounds__(kWG, LAT ? 1 : 3) void k_p2g_adj(ddm_batch B, const ddm_state *__restrict__ cur_tab,
                                                  const ddm_state *__restrict__ cur_grad_tab,
-                                                 const ddm_state *__restrict__ next_grad_tab) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;  // the grid kernels have finished
+                                                 const ddm_state *__restrict__ next_grad_tab,
+                                                 const ddm_scene *restore_prev) {
+  if (restore_prev && (int)blockIdx.x >= B.n_chunks) {  // uniform
+    ddm_batch Bp = B;
+    Bp.scenes = restore_prev;
+    const int r = (int)blockIdx.x - B.n_chunks;
+    restore_scene_part(Bp, r / kRestoreWGs, r % kRestoreWGs, kRestoreWGs);
+    return;
+  }
+  // the grid kernels have finished (a restore running beside sets the count itself)
+  if (!restore_prev && blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = 0;
   const int ci = chunk_index(B);
   if (ci < 0) return;  // uniform: a fixed-size launch with fewer chunks
   const int4 ch = reinterpret_cast<const int4 *>(B.chunks)[ci];
@@ -414,9 +426,11 @@ void launch_grid_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *
   k_det_body_reduce<<<b.n_scenes, 64, 0, stream>>>(b, d, cur_grad, next_grad);
 }
 void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
-                    hipStream_t stream) {
-  if (b.n_chunks <= kLatencyChunks) k_p2g_adj<true><<<b.n_chunks, kWG, 0, stream>>>(b, cur, cur_grad, next_grad);
-  else k_p2g_adj<false><<<b.n_chunks, kWG, 0, stream>>>(b, cur, cur_grad, next_grad);
+                    hipStream_t stream, const ddm_scene *restore_prev) {
+  const int wgs = b.n_chunks + (restore_prev ? kRestoreWGs * b.n_scenes : 0);
+  if (b.n_chunks <= kLatencyChunks)
+    k_p2g_adj<true><<<wgs, kWG, 0, stream>>>(b, cur, cur_grad, next_grad, restore_prev);
+  else k_p2g_adj<false><<<wgs, kWG, 0, stream>>>(b, cur, cur_grad, next_grad, restore_prev);
 }
 
 }  // namespace ddm_fused

@@ -607,6 +607,56 @@ __device__ __forceinline__ int chunk_index(const ddm_batch &B) {
 }
 __device__ __forceinline__ int live_scenes(const ddm_batch &B) { return B.live ? B.live[1] : B.n_scenes; }
 
+// blocks a scene restores from its snapshot (0 without one, or when it overflowed)
+__device__ __forceinline__ int restorable(const ddm_scene &S) {
+  if (!S.tape_mmv) return 0;
+  const int n = S.tape_status[0];
+  return n <= S.tape_capacity ? n : 0;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Taped adjoint: for every scene with a snapshot that holds all its listed blocks, puts
+// the blocks back into the arena and into the block list (scenes in order, each at the
+// prefix sum of the restored counts before it), and clears tape_status[1]; a scene whose
+// forward listed more blocks than its snapshot holds gets tape_status[1] = 1 and is
+// recomputed by the kernels that follow (its arena blocks are untouched, i.e. clear; its
+// recomputed blocks are appended after the restored ones). Run by kRestoreWGs workgroups
+// per scene: this is workgroup `part` of `parts` for scene si (k_restore's body; the taped
+// adjoint's P2G adjoint can also run it, in extra workgroups, for the substep before).
+constexpr int kRestoreWGs = 16;
+__device__ void restore_scene_part(const ddm_batch &B, int si, int part, int parts) {
+  const int n_live = live_scenes(B);
+  if (si >= n_live) return;
+  const ddm_scene &S = B.scenes[si];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // restored entries of the scenes before this one, and of all scenes
+  int before = 0, total = 0;
+  for (int s0 = 0; s0 < n_live; s0 += 64) {
+    const int s2 = s0 + lane;
+    const int c = s2 < n_live ? restorable(B.scenes[s2]) : 0;
+    total += wave_sum_i(c);
+    before += wave_sum_i(s2 < si ? c : 0);
+  }
+  if (si == 0 && part == 0 && threadIdx.x == 0) *B.block_count = total;
+  if (!S.tape_mmv) return;
+  const int n = S.tape_status[0];
+  const bool use = n <= S.tape_capacity;
+  if (part == 0 && threadIdx.x == 0) S.tape_status[1] = use ? 0 : 1;
+  if (!use) return;
+  int2 *list = reinterpret_cast<int2 *>(B.block_list) + before;
+  for (int e = part * (kWG / 64) + wave; e < n; e += parts * (kWG / 64)) {
+    const int blk = S.tape_list[e];
+    const size_t gi = ((size_t)blk << 6) | (size_t)lane, ti = ((size_t)e << 6) | (size_t)lane;
+    reinterpret_cast<float4 *>(B.grid_mmv)[gi] = reinterpret_cast<const float4 *>(S.tape_mmv)[ti];
+    reinterpret_cast<float4 *>(B.grid_v)[gi] = reinterpret_cast<const float4 *>(S.tape_v)[ti];
+    if (lane == 0) list[e] = make_int2(blk, si);
+  }
+}
+
 }  // namespace
 
 // Adjoint kernels (fused_adjoint.hip, compiled with FMA contraction: they feed only
@@ -617,7 +667,8 @@ void launch_g2p_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *n
 void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                      const ddm_state *next_grad, hipStream_t stream);
 void launch_p2g_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *cur_grad, const ddm_state *next_grad,
-                    hipStream_t stream);
+                    hipStream_t stream,
+                    const ddm_scene *restore_prev = nullptr);
 // deterministic mode: G2P adjoint (per-particle part, then the sorted node gather) and the
 // grid adjoint with per-block body-pose partials reduced in block order
 void launch_g2p_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,

@@ -210,53 +210,8 @@ __global__ __launch_bounds__(kWG) void k_grid(ddm_batch B, const ddm_state *__re
 }
 
 
-// blocks a scene restores from its snapshot (0 without one, or when it overflowed)
-__device__ __forceinline__ int restorable(const ddm_scene &S) {
-  if (!S.tape_mmv) return 0;
-  const int n = S.tape_status[0];
-  return n <= S.tape_capacity ? n : 0;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Taped adjoint: for every scene with a snapshot that holds all its listed blocks, puts
-// the blocks back into the arena and into the block list (scenes in order, each at the
-// prefix sum of the restored counts before it), and clears tape_status[1]; a scene whose
-// forward listed more blocks than its snapshot holds gets tape_status[1] = 1 and is
-// recomputed by the kernels that follow (its arena blocks are untouched, i.e. clear; its
-// recomputed blocks are appended after the restored ones). Grid: (kRestoreWGs, n_scenes),
-// one row of workgroups per scene.
-constexpr int kRestoreWGs = 16;
 __global__ __launch_bounds__(kWG) void k_restore(ddm_batch B) {
-  const int si = (int)blockIdx.y, n_live = live_scenes(B);
-  if (si >= n_live) return;
-  const ddm_scene &S = B.scenes[si];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // restored entries of the scenes before this one, and of all scenes
-  int before = 0, total = 0;
-  for (int s0 = 0; s0 < n_live; s0 += 64) {
-    const int s2 = s0 + lane;
-    const int c = s2 < n_live ? restorable(B.scenes[s2]) : 0;
-    total += wave_sum_i(c);
-    before += wave_sum_i(s2 < si ? c : 0);
-  }
-  if (si == 0 && blockIdx.x == 0 && threadIdx.x == 0) *B.block_count = total;
-  if (!S.tape_mmv) return;
-  const int n = S.tape_status[0];
-  const bool use = n <= S.tape_capacity;
-  if (blockIdx.x == 0 && threadIdx.x == 0) S.tape_status[1] = use ? 0 : 1;
-  if (!use) return;
-  int2 *list = reinterpret_cast<int2 *>(B.block_list) + before;
-  for (int e = blockIdx.x * (kWG / 64) + wave; e < n; e += gridDim.x * (kWG / 64)) {
-    const int blk = S.tape_list[e];
-    const size_t gi = ((size_t)blk << 6) | (size_t)lane, ti = ((size_t)e << 6) | (size_t)lane;
-    reinterpret_cast<float4 *>(B.grid_mmv)[gi] = reinterpret_cast<const float4 *>(S.tape_mmv)[ti];
-    reinterpret_cast<float4 *>(B.grid_v)[gi] = reinterpret_cast<const float4 *>(S.tape_v)[ti];
-    if (lane == 0) list[e] = make_int2(blk, si);
-  }
+  restore_scene_part(B, (int)blockIdx.y, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------- G2P
@@ -607,7 +562,7 @@ static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_st
 // adjoint kernels after the grid update (both adjoint launch sequences)
 static void launch_adjoint_tail(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                 const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
-                                hipEvent_t *ev, int ev0) {
+                                hipEvent_t *ev, int ev0, const ddm_scene *restore_prev = nullptr) {
   if (b->options & DDM_DETERMINISTIC) {
     launch_g2p_adj_det(*b, *b->det, cur, next, cur_grad, next_grad, stream);
     mark(ev, ev0, stream);
@@ -618,7 +573,7 @@ static void launch_adjoint_tail(const ddm_batch *b, const ddm_state *cur, const 
     launch_grid_adj(*b, cur, next, cur_grad, next_grad, stream);
   }
   mark(ev, ev0 + 1, stream);
-  launch_p2g_adj(*b, cur, cur_grad, next_grad, stream);
+  launch_p2g_adj(*b, cur, cur_grad, next_grad, stream, restore_prev);
   mark(ev, ev0 + 2, stream);
 }
 
@@ -658,13 +613,14 @@ static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const 
 
 static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                  const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream,
-                                 hipEvent_t *ev) {
+                                 hipEvent_t *ev, int restored = 0, const ddm_scene *restore_prev = nullptr) {
   int e = check(b, cur, next);
   if (e) return e;
   if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
+  if ((restored || restore_prev) && (b->options & DDM_DETERMINISTIC)) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
-  k_restore<<<dim3(kRestoreWGs, b->n_scenes), kWG, 0, stream>>>(*b);
+  if (!restored) k_restore<<<dim3(kRestoreWGs, b->n_scenes), kWG, 0, stream>>>(*b);
   // With a complete tape every scene restores its whole substep and both recompute kernels
   // would exit at once; the deterministic recompute P2G also rebuilds the per-cell particle
   // order that the sorted adjoint gathers read, so it always runs.
@@ -674,7 +630,7 @@ static int launch_backward_taped(const ddm_batch *b, const ddm_state *cur, const
     k_grid<false, false, true><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   }
   mark(ev, 1, stream);
-  launch_adjoint_tail(b, cur, next, cur_grad, next_grad, stream, ev, 2);
+  launch_adjoint_tail(b, cur, next, cur_grad, next_grad, stream, ev, 2, restore_prev);
   return (int)hipGetLastError();
 }
 
@@ -686,6 +642,12 @@ DDM_API int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, 
 DDM_API int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                        const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
   return launch_backward_taped(b, cur, next, cur_grad, next_grad, stream, nullptr);
+}
+
+DDM_API int ddm_substep_backward_taped_ahead(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                             const ddm_state *cur_grad, const ddm_state *next_grad, int restored,
+                                             const ddm_scene *prev_scenes, hipStream_t stream) {
+  return launch_backward_taped(b, cur, next, cur_grad, next_grad, stream, nullptr, restored, prev_scenes);
 }
 
 DDM_API int ddm_substep_forward_taped_timed(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,

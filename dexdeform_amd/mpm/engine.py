@@ -429,14 +429,23 @@ class Engine:
                                                       _lib.stream_ptr(stream)), "ddm_substep_forward_taped")
 
     def backward_taped(self, cur_tab, nxt_tab, gcur_tab, gnxt_tab, scene_tab, stream=None, overwrite=False,
-                       complete=False):
+                       complete=False, restored=False, prev_scene_tab=None):
         """Adjoint substep from what forward_taped recorded with the same scene table.
         complete=True: every scene's snapshot holds its whole grid (tape_complete), so the
-        recompute kernels for overflowing scenes are not launched."""
+        recompute kernels for overflowing scenes are not launched. prev_scene_tab (the
+        previous substep's scene table): this substep's P2G adjoint launch also puts that
+        substep's snapshots back, and its own call then passes restored=True (not in the
+        deterministic mode)."""
         if not self.with_grad:
             raise RuntimeError("engine was created with with_grad=False")
         opts = (_lib.DDM_GRAD_OVERWRITE if overwrite else 0) | (_lib.DDM_TAPE_COMPLETE if complete else 0)
         b = self._batch_with(scene_tab, opts)
+        if restored or prev_scene_tab is not None:
+            prev = prev_scene_tab.data_ptr() if prev_scene_tab is not None else None
+            _lib.check(self.lib.ddm_substep_backward_taped_ahead(
+                ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(), gcur_tab.data_ptr(), gnxt_tab.data_ptr(),
+                int(bool(restored)), prev, _lib.stream_ptr(stream)), "ddm_substep_backward_taped_ahead")
+            return
         _lib.check(self.lib.ddm_substep_backward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                        gcur_tab.data_ptr(), gnxt_tab.data_ptr(),
                                                        _lib.stream_ptr(stream)), "ddm_substep_backward_taped")

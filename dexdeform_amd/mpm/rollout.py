@@ -209,6 +209,7 @@ class Rollout:
                 for j in range(self.K):
                     self._fwd(j, base + j, taped=self.tape is not None)
                 self._last_segment_taped = False
+            restored = False
             for j in range(self.K - 1, -1, -1):
                 t = base + j
                 nxt_slot = cur_slot
@@ -217,7 +218,12 @@ class Rollout:
                         self._gview(nxt_slot, t + 1))
                 # the adjoint substep writes (not adds to) the state-t gradient: no clearing
                 if self.tape is not None:
-                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], overwrite=True, complete=self.tape_complete)
+                    # within a segment, each substep's P2G adjoint launch also restores the
+                    # snapshot of the substep before (whose call then skips its restore)
+                    prev = self.tape.scene_tabs[j - 1] if j > 0 and not eng.deterministic else None
+                    eng.backward_taped(*tabs, self.tape.scene_tabs[j], overwrite=True, complete=self.tape_complete,
+                                       restored=restored, prev_scene_tab=prev)
+                    restored = prev is not None
                 else:
                     eng.backward_tables(*tabs, overwrite=True)
         self._g0 = self.g[cur_slot]

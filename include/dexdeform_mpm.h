@@ -293,6 +293,18 @@ int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, const dd
 int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream);
 
+/* ddm_substep_backward_taped in a sequence of substeps (t, then t-1, ...): with
+ * prev_scenes (the device scene table of substep t-1, recorded by its taped forward with
+ * the same batch) the P2G adjoint launch of substep t also puts back substep t-1's
+ * snapshots, in extra workgroups, and the call for substep t-1 then passes restored = 1
+ * and launches no restore of its own: one launch fewer per substep. prev_scenes NULL and
+ * restored 0 is ddm_substep_backward_taped. Not with DDM_DETERMINISTIC (DDM_EINVAL). No
+ * counterpart in the reference; for callers that run the taped adjoint substep after
+ * substep, such as rollouts over a checkpoint segment. */
+int ddm_substep_backward_taped_ahead(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                     const ddm_state *cur_grad, const ddm_state *next_grad, int restored,
+                                     const ddm_scene *prev_scenes, hipStream_t stream);
+
 /* The taped substeps with hipEvents recorded between their stages (as the _timed
  * variants above): forward events[0..3] around P2G, grid update, G2P; backward events[0..4]
  * around restore (+ recompute where needed), G2P adjoint, grid adjoint, P2G adjoint. */
