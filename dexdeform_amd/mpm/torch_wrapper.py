@@ -228,14 +228,18 @@ class GradModel:
         eng = sim.engine
 
         def run():
+            restored = False
             for i in range(f + S - 1, f - 1, -1):
                 cur, nxt = sim.states[i], sim.states[i + 1]
                 if tapes is None:
                     eng.backward(cur.buf, nxt.buf, cur.grad, nxt.grad, overwrite=True)
                 else:
+                    # each substep's P2G adjoint launch also restores the step's substep before
+                    prev = tapes.scene_tabs[i - 1] if i > f and not eng.deterministic else None
                     eng.backward_taped(eng.table_of(cur.buf), eng.table_of(nxt.buf), eng.table_of(cur.grad),
                                        eng.table_of(nxt.grad), tapes.scene_tabs[i], overwrite=True,
-                                       complete=tapes.complete)
+                                       complete=tapes.complete, restored=restored, prev_scene_tab=prev)
+                    restored = prev is not None
             # the pose gradients of states f+1 .. f+S are complete once the step's adjoint
             # substeps have run (later adjoint substeps touch state f and earlier only)
             torch._foreach_copy_(dst, src)
