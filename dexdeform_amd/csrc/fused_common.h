@@ -48,10 +48,10 @@ constexpr int kTapeMax = 8;      // contact tape entries kept per node in the ad
 // issued together; as many registers as one wave per SIMD allows).
 constexpr int kLatencyChunks = 256;
 // Forward launches of at most kLatencyChunks chunks over at most this many node blocks
-// skip the block list: P2G marks its blocks with plain stores of scene + 1 and the grid
-// update scans every block's flag (one wave per block). The list append costs P2G two
-// dependent device-scope atomics at its end (after its own tile flush has drained),
-// which such a launch waits for; the scan costs the grid update one flag load per wave.
+// skip the block list: P2G marks its blocks with plain stores of scene + 1 instead of two
+// dependent device-scope atomics with return (which mostly overlap the drain of its own
+// tile flush), and the grid update scans every block's flag, one wave per block, instead
+// of reading a list entry and then the scene: one dependent load fewer on its chain.
 constexpr int kScanBlocks = 16384;
 
 __device__ __forceinline__ int node_index(const ddm_scene &S, int x, int y, int z) {
