@@ -108,7 +108,7 @@ REFERENCE_SYMBOLS = ["compute_grid_lower", "compute_svd", "compute_svd_grad", "p
 FUSED_SYMBOLS = ["ddm_substep_forward", "ddm_substep_backward", "ddm_substep_forward_timed",
                  "ddm_substep_backward_timed", "ddm_substep_forward_taped", "ddm_substep_backward_taped",
                  "ddm_substep_forward_taped_timed", "ddm_substep_backward_taped_timed", "ddm_batch_reset",
-                 "ddm_substep_backward_taped_ahead",
+                 "ddm_substep_backward_taped_ahead", "ddm_substep_forward_ahead",
                  "ddm_version", "ddm_substep_forward_scratch"]
 SCORE_SYMBOLS = ["ddm_pairwise_workspace_bytes", "ddm_softmin", "ddm_plan_grad", "ddm_nearest"]
 RENDER_SYMBOLS = ["particle_sdf", "render", "create_volume", "destroy_volume"]
@@ -150,6 +150,7 @@ _ARGTYPES = {
     "ddm_substep_forward_taped": [ctypes.POINTER(DdmBatch), _P, _P, _P],
     "ddm_substep_backward_taped": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P],
     "ddm_substep_backward_taped_ahead": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, ctypes.c_int, _P, _P],
+    "ddm_substep_forward_ahead": [ctypes.POINTER(DdmBatch), _P, _P, _P, ctypes.c_int, ctypes.c_int, _P],
     "ddm_substep_forward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P],
     "ddm_substep_backward_taped_timed": [ctypes.POINTER(DdmBatch), _P, _P, _P, _P, _P, _P],
     "ddm_batch_reset": [ctypes.POINTER(DdmBatch), _P],

@@ -25,9 +25,14 @@ __device__ __forceinline__ void tape_reset(const ddm_batch &B) {
 }
 
 // at most 96 VGPRs (a few bytes of spill): five waves per SIMD, as the LDS allows
-template <int MODE>
-__global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
-                                             const ddm_state *__restrict__ next_tab, int scan) {
+// AHEAD (small forward launches): each particle first runs the G2P of the substep before
+// (prev_tab -> cur_tab; the grid update of that substep has finished) and P2G then uses
+// the x, v, C it computed instead of reading them back: one launch fewer per substep.
+template <int MODE, bool AHEAD = false>
+__global__ __launch_bounds__(kWG, AHEAD ? 1 : 5) void k_p2g(ddm_batch B, const ddm_state *__restrict__ cur_tab,
+                                                          const ddm_state *__restrict__ next_tab, int scan,
+                                                          const ddm_state *__restrict__ prev_tab = nullptr) {
+  static_assert(!AHEAD || MODE == kForward || MODE == kForwardTaped, "G2P ahead of forward P2G only");
   __shared__ ScatterLDS<4, kTileP2G> sh;
   __shared__ StencilRecs rc;
   __shared__ Groups gr;
@@ -49,9 +54,19 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
   P2GParticle q;
   int key = 0;
   if (valid) {
+    f3 xa, va;
+    m33 Ca;
+    if (AHEAD) {  // G2P of the substep before, as k_g2p computes it
+      BlockedV gv{reinterpret_cast<const float4 *>(B.grid_v), &S};
+      g2p_particle_prefetch(ld3(prev_tab[ch.x].x + 3 * (size_t)p), gv, S.inv_dx, S.dx, S.dt, S.ground_height, dim,
+                            xa, va, Ca);
+      st3(cur.x + 3 * (size_t)p, xa);
+      st3(cur.v + 3 * (size_t)p, va);
+      st9(cur.C + 9 * (size_t)p, Ca);
+    }
     const float *Cp = cur.C + 9 * (size_t)p;
     m33 F = ld9(cur.F + 9 * (size_t)p);
-    m33 Ft = advect_F(F, ld9(Cp), S.dt);
+    m33 Ft = advect_F(F, AHEAD ? Ca : ld9(Cp), S.dt);
     m33 U, V;
     f3 sig;
     q4 qu, qv;
@@ -61,10 +76,11 @@ __global__ __launch_bounds__(kWG, 5) void k_p2g(ddm_batch B, const ddm_state *__
       stq(S.tmp_qv + 4 * (size_t)p, qv);
       st3(S.tmp_sig + 3 * (size_t)p, sig);
     }
-    asm volatile("" : "+v"(Cp));  // C is read again (from the cache) rather than held across the SVD
+    if (!AHEAD) asm volatile("" : "+v"(Cp));  // C is read again (from the cache) rather than held across the SVD
     m33 Fn;
-    p2g_particle(ld3(cur.x + 3 * (size_t)p), ld3(cur.v + 3 * (size_t)p), ld9(Cp), Ft, U, sig, V, S.mass[p], S.vol[p],
-                 ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx, S.inv_dx, S.dt, dim, Fn, q);
+    p2g_particle(AHEAD ? xa : ld3(cur.x + 3 * (size_t)p), AHEAD ? va : ld3(cur.v + 3 * (size_t)p),
+                 AHEAD ? Ca : ld9(Cp), Ft, U, sig, V, S.mass[p], S.vol[p], ld3(S.mu_lam_yield + 3 * (size_t)p), S.dx,
+                 S.inv_dx, S.dt, dim, Fn, q);
     if (MODE == kForward || MODE == kForwardTaped) st9(next_tab[ch.x].F + 9 * (size_t)p, Fn);
     key = pack_base(q.st.base);
   } else {
@@ -541,20 +557,25 @@ static inline void mark(hipEvent_t *ev, int i, hipStream_t stream) {
   if (ev) (void)hipEventRecord(ev[i], stream);
 }
 
+// prev (optional): the G2P of the substep before (prev -> cur) has not run yet; it runs
+// first, inside this substep's P2G launch where the launch scans the block flags, else as
+// its own launch. finish = 0: this substep's G2P is left to the next call (given prev).
 static int launch_forward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, hipStream_t stream,
-                          hipEvent_t *ev) {
+                          hipEvent_t *ev, const ddm_state *prev = nullptr, int finish = 1) {
   int e = check(b, cur, next);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
   const bool det = b->options & DDM_DETERMINISTIC, scan = scan_blocks(b);
+  if (prev && !scan) k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, prev, cur);
   mark(ev, 0, stream);
   if (det) launch_det_p2g<kForward>(b, cur, next, stream);
+  else if (prev && scan) k_p2g<kForward, true><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, scan, prev);
   else k_p2g<kForward><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, scan);
   mark(ev, 1, stream);
   if (scan) k_grid<true, false, false, true><<<scan_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   else k_grid<true, false, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
   mark(ev, 2, stream);
-  k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  if (finish) k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
   return (int)hipGetLastError();
 }
@@ -594,19 +615,22 @@ static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_s
 }
 
 static int launch_forward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
-                                hipStream_t stream, hipEvent_t *ev) {
+                                hipStream_t stream, hipEvent_t *ev, const ddm_state *prev = nullptr,
+                                int finish = 1) {
   int e = check(b, cur, next);
   if (e) return e;
   if (b->n_chunks == 0) return 0;
   const bool det = b->options & DDM_DETERMINISTIC, scan = scan_blocks(b);
+  if (prev && !scan) k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, prev, cur);
   mark(ev, 0, stream);
   if (det) launch_det_p2g<kForwardTaped>(b, cur, next, stream);
+  else if (prev && scan) k_p2g<kForwardTaped, true><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, scan, prev);
   else k_p2g<kForwardTaped><<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next, scan);
   mark(ev, 1, stream);
   if (scan) k_grid<true, true, false, true><<<scan_wgs(b), kWG, 0, stream>>>(*b, cur, next, nullptr);
   else k_grid<true, true, false><<<grid_wgs(b), kWG, 0, stream>>>(*b, cur, next, det ? b->det->cell_count : nullptr);
   mark(ev, 2, stream);
-  k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
+  if (finish) k_g2p<<<b->n_chunks, kWG, 0, stream>>>(*b, cur, next);
   mark(ev, 3, stream);
   return (int)hipGetLastError();
 }
@@ -642,6 +666,13 @@ DDM_API int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, 
 DDM_API int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                        const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {
   return launch_backward_taped(b, cur, next, cur_grad, next_grad, stream, nullptr);
+}
+
+DDM_API int ddm_substep_forward_ahead(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                                      const ddm_state *prev, int finish, int taped, hipStream_t stream) {
+  if (!finish && !next) return DDM_EINVAL;
+  return taped ? launch_forward_taped(b, cur, next, stream, nullptr, prev, finish)
+               : launch_forward(b, cur, next, stream, nullptr, prev, finish);
 }
 
 DDM_API int ddm_substep_backward_taped_ahead(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,

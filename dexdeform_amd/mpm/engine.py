@@ -343,7 +343,16 @@ class Engine:
         self.backward_tables(self.table_of(cur), self.table_of(nxt), self.table_of(gcur), self.table_of(gnxt), stream,
                              overwrite)
 
-    def forward_tables(self, cur_tab, nxt_tab, stream=None):
+    def forward_tables(self, cur_tab, nxt_tab, stream=None, prev_tab=None, finish=True):
+        """One forward substep. In a sequence of substeps: finish=False leaves this
+        substep's G2P to the next call, which passes prev_tab (this call's cur_tab) and
+        runs it first, inside its own P2G launch for small launches."""
+        if prev_tab is not None or not finish:
+            _lib.check(self.lib.ddm_substep_forward_ahead(
+                ctypes.byref(self._batch), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+                prev_tab.data_ptr() if prev_tab is not None else None, int(bool(finish)), 0,
+                _lib.stream_ptr(stream)), "ddm_substep_forward_ahead")
+            return
         _lib.check(self.lib.ddm_substep_forward(ctypes.byref(self._batch), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                 _lib.stream_ptr(stream)), "ddm_substep_forward")
 
@@ -421,10 +430,16 @@ class Engine:
     def _batch_with_scenes(self, scene_tab):
         return self._batch_with(scene_tab)
 
-    def forward_taped(self, cur_tab, nxt_tab, scene_tab, stream=None):
+    def forward_taped(self, cur_tab, nxt_tab, scene_tab, stream=None, prev_tab=None, finish=True):
         """Forward substep recording, per scene, the snapshot and temporaries that the
-        scene table from taped_table points to."""
+        scene table from taped_table points to. prev_tab / finish as in forward_tables."""
         b = self._batch_with_scenes(scene_tab)
+        if prev_tab is not None or not finish:
+            _lib.check(self.lib.ddm_substep_forward_ahead(
+                ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
+                prev_tab.data_ptr() if prev_tab is not None else None, int(bool(finish)), 1,
+                _lib.stream_ptr(stream)), "ddm_substep_forward_ahead")
+            return
         _lib.check(self.lib.ddm_substep_forward_taped(ctypes.byref(b), cur_tab.data_ptr(), nxt_tab.data_ptr(),
                                                       _lib.stream_ptr(stream)), "ddm_substep_forward_taped")
 

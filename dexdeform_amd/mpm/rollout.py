@@ -161,17 +161,23 @@ class Rollout:
         return self.seg[self.K] if self.full else self._final
 
     # ------------------------------------------------------------------ passes
-    def _fwd(self, j, t, taped=False):
+    def _fwd(self, j, t, taped=False, ahead=False, finish=True):
+        """Forward substep t from segment state j. ahead: the G2P of substep t-1 (state j-1
+        -> j) was left to this call; finish=False leaves this substep's G2P to the next."""
+        prev = self._sview(j - 1, t - 1) if ahead else None
         if taped:
-            self.engine.forward_taped(self._sview(j, t), self._sview(j + 1, t + 1), self.tape.scene_tabs[j])
+            self.engine.forward_taped(self._sview(j, t), self._sview(j + 1, t + 1), self.tape.scene_tabs[j],
+                                      prev_tab=prev, finish=finish)
         else:
-            self.engine.forward_tables(self._sview(j, t), self._sview(j + 1, t + 1))
+            self.engine.forward_tables(self._sview(j, t), self._sview(j + 1, t + 1), prev_tab=prev, finish=finish)
 
     def forward(self):
         """T forward substeps from the initial state."""
+        # within a run of substeps each G2P runs inside the next substep's P2G launch
+        # (ahead=True there, finish=False here); the run's last substep finishes its own
         if self.full:
             for t in range(self.T):
-                self._fwd(t, t, taped=self.tape is not None)
+                self._fwd(t, t, taped=self.tape is not None, ahead=t > 0, finish=t == self.T - 1)
             return self.seg[self.T]
         # checkpointed: run each segment in the segment buffer and keep its end state;
         # segment 0 restarts from checkpoint 0 too, since a previous pass left a later
@@ -182,7 +188,7 @@ class Rollout:
             self.seg[0].copy_(self.ckpt[c])
             last = c == n_seg - 1 and self.tape is not None
             for j in range(self.K):
-                self._fwd(j, c * self.K + j, taped=last)
+                self._fwd(j, c * self.K + j, taped=last, ahead=j > 0, finish=j == self.K - 1)
             self.ckpt[c + 1].copy_(self.seg[self.K])
         self._last_segment_taped = self.tape is not None
         self._final = self.ckpt[-1]
@@ -207,7 +213,7 @@ class Rollout:
             if not self.full and not (c == n_seg - 1 and self._last_segment_taped):
                 self.seg[0].copy_(self.ckpt[c])
                 for j in range(self.K):
-                    self._fwd(j, base + j, taped=self.tape is not None)
+                    self._fwd(j, base + j, taped=self.tape is not None, ahead=j > 0, finish=j == self.K - 1)
                 self._last_segment_taped = False
             restored = False
             for j in range(self.K - 1, -1, -1):

@@ -293,6 +293,17 @@ int ddm_substep_forward_taped(const ddm_batch *b, const ddm_state *cur, const dd
 int ddm_substep_backward_taped(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
                                const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream);
 
+/* ddm_substep_forward (taped = 0) or ddm_substep_forward_taped (taped = 1, b's scene
+ * table that of this substep) in a sequence of substeps (t-1, then t, ...): finish = 0
+ * leaves this substep's G2P (cur -> next) to the next call, which passes prev = this
+ * call's cur and runs that G2P first (prev -> its cur), inside its P2G launch where the
+ * launch is small enough to scan block flags, else as its own launch. One launch fewer
+ * per substep for small scenes. prev NULL and finish 1 is the plain substep; the last
+ * substep of a sequence passes finish = 1. No counterpart in the reference; for callers
+ * that run forward substeps in sequence, such as rollouts over a checkpoint segment. */
+int ddm_substep_forward_ahead(const ddm_batch *b, const ddm_state *cur, const ddm_state *next,
+                              const ddm_state *prev, int finish, int taped, hipStream_t stream);
+
 /* ddm_substep_backward_taped in a sequence of substeps (t, then t-1, ...): with
  * prev_scenes (the device scene table of substep t-1, recorded by its taped forward with
  * the same batch) the P2G adjoint launch of substep t also puts back substep t-1's
