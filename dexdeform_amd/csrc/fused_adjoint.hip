@@ -10,7 +10,9 @@ namespace {
 // DET (deterministic mode): each block writes its body-pose gradient sums to its list
 // entry's partials (k_det_body_reduce adds them up in block order) instead of atomics,
 // and clears its bin counts.
-template <bool DET>
+// SCAN (complete snapshots, small launches): every block of the launch is visited and those
+// the restore marked (with scene + 1) are processed, instead of reading the block list.
+template <bool DET, bool SCAN = false>
 __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const ddm_state *__restrict__ cur_tab,
                                                   const ddm_state *__restrict__ next_tab,
                                                   const ddm_state *__restrict__ cur_grad_tab,
@@ -20,11 +22,21 @@ __global__ __launch_bounds__(kWG) void k_grid_adj(ddm_batch B, ddm_det D, const 
   float4 *mmv = reinterpret_cast<float4 *>(B.grid_mmv);
   float4 *dv = reinterpret_cast<float4 *>(B.grid_dv);
   float4 *dmmv = reinterpret_cast<float4 *>(B.grid_dmmv);
-  const int n_listed = *B.block_count;
+  static_assert(!(DET && SCAN), "the deterministic mode reduces in list order");
+  const int n_items = SCAN ? B.n_blocks : *B.block_count;
   const int2 *list = reinterpret_cast<const int2 *>(B.block_list);
-  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_listed; e += gridDim.x * (kWG / 64)) {
-    const int2 entry = list[e];
-    const int blk = entry.x, si = entry.y;
+  for (int e = blockIdx.x * (kWG / 64) + wave; e < n_items; e += gridDim.x * (kWG / 64)) {
+    int blk, si;
+    if (SCAN) {
+      const uint32_t f = B.block_flags[e];
+      if (f == 0u) continue;  // the same on every lane
+      blk = e;
+      si = (int)f - 1;
+    } else {
+      const int2 entry = list[e];
+      blk = entry.x;
+      si = entry.y;
+    }
     const ddm_scene &S = B.scenes[si];
     BlockCoord c = block_node(S, blk, lane);
     size_t gi = ((size_t)blk << 6) | (size_t)lane;
@@ -413,7 +425,9 @@ void launch_g2p_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *n
 }
 void launch_grid_adj(const ddm_batch &b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
                      const ddm_state *next_grad, hipStream_t stream) {
-  k_grid_adj<false><<<grid_wgs(&b), kWG, 0, stream>>>(b, ddm_det{}, cur, next, cur_grad, next_grad);
+  if ((b.options & DDM_TAPE_COMPLETE) && scan_blocks(&b))  // every listed block was restored, and marked
+    k_grid_adj<false, true><<<scan_wgs(&b), kWG, 0, stream>>>(b, ddm_det{}, cur, next, cur_grad, next_grad);
+  else k_grid_adj<false><<<grid_wgs(&b), kWG, 0, stream>>>(b, ddm_det{}, cur, next, cur_grad, next_grad);
 }
 void launch_g2p_adj_det(const ddm_batch &b, const ddm_det &d, const ddm_state *cur, const ddm_state *next,
                         const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream) {

@@ -624,9 +624,18 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // prefix sum of the restored counts before it), and clears tape_status[1]; a scene whose
 // forward listed more blocks than its snapshot holds gets tape_status[1] = 1 and is
 // recomputed by the kernels that follow (its arena blocks are untouched, i.e. clear; its
-// recomputed blocks are appended after the restored ones). Run by kRestoreWGs workgroups
+// recomputed blocks are appended after the restored ones), and marks each restored block
+// with scene + 1 (outside the deterministic mode). Run by kRestoreWGs workgroups
 // per scene: this is workgroup `part` of `parts` for scene si (k_restore's body; the taped
 // adjoint's P2G adjoint can also run it, in extra workgroups, for the substep before).
+// launches whose grid kernels visit every block's flag instead of the block list
+// (non-deterministic, few chunks, few blocks)
+inline bool scan_blocks(const ddm_batch *b) {
+  return !(b->options & DDM_DETERMINISTIC) && b->n_chunks <= kLatencyChunks && b->n_blocks > 0 &&
+         b->n_blocks <= kScanBlocks;
+}
+inline int scan_wgs(const ddm_batch *b) { return (b->n_blocks + kWG / 64 - 1) / (kWG / 64); }
+
 constexpr int kRestoreWGs = 16;
 __device__ void restore_scene_part(const ddm_batch &B, int si, int part, int parts) {
   const int n_live = live_scenes(B);
@@ -653,7 +662,11 @@ __device__ void restore_scene_part(const ddm_batch &B, int si, int part, int par
     const size_t gi = ((size_t)blk << 6) | (size_t)lane, ti = ((size_t)e << 6) | (size_t)lane;
     reinterpret_cast<float4 *>(B.grid_mmv)[gi] = reinterpret_cast<const float4 *>(S.tape_mmv)[ti];
     reinterpret_cast<float4 *>(B.grid_v)[gi] = reinterpret_cast<const float4 *>(S.tape_v)[ti];
-    if (lane == 0) list[e] = make_int2(blk, si);
+    if (lane == 0) {
+      list[e] = make_int2(blk, si);
+      // the mark a scanning grid adjoint reads (it clears it, as the listing one does)
+      if (!(B.options & DDM_DETERMINISTIC)) B.block_flags[blk] = (uint32_t)(si + 1);
+    }
   }
 }
 

@@ -546,13 +546,6 @@ using namespace ddm_fused;
 
 #define DDM_API extern "C" __attribute__((visibility("default")))
 
-// forward launches that skip the block list (non-deterministic, few chunks, few blocks)
-static inline bool scan_blocks(const ddm_batch *b) {
-  return !(b->options & DDM_DETERMINISTIC) && b->n_chunks <= kLatencyChunks && b->n_blocks > 0 &&
-         b->n_blocks <= kScanBlocks;
-}
-static inline int scan_wgs(const ddm_batch *b) { return (b->n_blocks + kWG / 64 - 1) / (kWG / 64); }
-
 static inline void mark(hipEvent_t *ev, int i, hipStream_t stream) {
   if (ev) (void)hipEventRecord(ev[i], stream);
 }
@@ -598,10 +591,13 @@ static void launch_adjoint_tail(const ddm_batch *b, const ddm_state *cur, const 
   mark(ev, ev0 + 2, stream);
 }
 
-static int launch_backward(const ddm_batch *b, const ddm_state *cur, const ddm_state *next, const ddm_state *cur_grad,
-                           const ddm_state *next_grad, hipStream_t stream, hipEvent_t *ev) {
-  int e = check(b, cur, next);
+static int launch_backward(const ddm_batch *b_in, const ddm_state *cur, const ddm_state *next,
+                           const ddm_state *cur_grad, const ddm_state *next_grad, hipStream_t stream, hipEvent_t *ev) {
+  int e = check(b_in, cur, next);
   if (e) return e;
+  ddm_batch bb = *b_in;  // DDM_TAPE_COMPLETE means nothing here (the grid adjoint must read the list)
+  bb.options &= ~DDM_TAPE_COMPLETE;
+  const ddm_batch *b = &bb;
   if (!cur_grad || !next_grad || !b->grid_dv || !b->grid_dmmv) return DDM_EINVAL;
   if (b->n_chunks == 0) return 0;
   mark(ev, 0, stream);
