@@ -200,11 +200,14 @@ class GradModel:
             torch._foreach_copy_(dst, src)  # one launch for the environment step's poses
             torch._foreach_zero_(clear)
             for i in range(S):
+                # each G2P runs inside the next substep's P2G launch; the step's last
+                # substep finishes its own
+                cur, nxt = eng.table_of(sim.states[f + i].buf), eng.table_of(states[i].buf)
+                prev = eng.table_of(sim.states[f + i - 1].buf) if i > 0 else None
                 if tapes is None:
-                    eng.forward(sim.states[f + i].buf, states[i].buf)
+                    eng.forward_tables(cur, nxt, prev_tab=prev, finish=i == S - 1)
                 else:
-                    eng.forward_taped(eng.table_of(sim.states[f + i].buf), eng.table_of(states[i].buf),
-                                      tapes.scene_tabs[f + i])
+                    eng.forward_taped(cur, nxt, tapes.scene_tabs[f + i], prev_tab=prev, finish=i == S - 1)
         self._graph_run("F", f, run)
 
     def _substeps_backward(self, f):
